@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused Shan-Chen step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Default workload (the metric's D3Q19 case): D3Q19 MRT Shan-Chen, 256^3
+periodic, FP64 populations, E4 force, psi = 1 - exp(-n), one spherical droplet
+R = 40 (n_l = 2.0, n_g = 0.1, tanh width 2), synthetic initial field.
+Other workloads: d2q9_4096_e8 (64 droplets, D2Q9 4096^2 FP64 E8 BGK),
+d3q19_bgk_256 (same as default with BGK), d3q19_mrt_256_fp32s (FP32 shifted
+populations).
+
+Prints ONE JSON line. value = MLUPS = V * K / (wall time of the K timed steps)
+with the populations resident in HBM. roofline.achieved = algorithmic bytes
+per launch (2 Q sizeof(pop) per site: every population read once and written
+once) / the kernel's average duration from HIP events recorded on the
+library's stream around the K launches. cpu_baseline = the oracle's serial
+C++ restatement of the reference CTypes step on one host core, timed on a
+bounded sample (see "sample").
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, 'idea.deploy_amd'))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    'd3q19_mrt_256': dict(xi='D3Q19', es='D3E4', dims=(256, 256, 256), coll='mrt',
+                          prec='fp64', init='sphere3d'),
+    'd3q19_bgk_256': dict(xi='D3Q19', es='D3E4', dims=(256, 256, 256), coll='bgk',
+                          prec='fp64', init='sphere3d'),
+    'd3q19_mrt_256_fp32s': dict(xi='D3Q19', es='D3E4', dims=(256, 256, 256), coll='mrt',
+                                prec='fp32s', init='sphere3d'),
+    'd2q9_4096_e8': dict(xi='D2Q9', es='D2E8', dims=(4096, 4096), coll='bgk', prec='fp64',
+                         init='droplets2d'),
+}
+
+
+def mrt_S(xi):
+    """Relaxation rates of the MRT workloads: omega = 1/tau (tau = 1) on the
+    second-order (shear and bulk) moments and 1 on all others."""
+    from idmx.LBM import hermite
+    _, tuples = hermite.invertible_set(xi)
+    return np.diag([1.0 for _ in tuples])
+
+
+def workload_desc(name, w):
+    d = {'workload': name, 'lattice': w['xi'], 'dims': list(w['dims']),
+         'force_stencil': w['es'], 'collision': w['coll'], 'psi': '1 - exp(-n)',
+         'SC_G': -5.0 / 3.0, 'tau': 1.0, 'populations': w['prec']}
+    if w['init'] == 'sphere3d':
+        d['init'] = 'sphere R=40, n_l=2.0, n_g=0.1, tanh width 2'
+    else:
+        d['init'] = '64 seeded droplets R in [10,60], n_l=2.0, n_g=0.1'
+    return d
+
+
+def run_gpu(name, steps, warmup):
+    from idmx import configs
+    from idmx.IdpyCode import IdpyMemory, Tenet
+    from idmx.LBM import hermite
+    from idmx.LBM._base import PopArray
+    from idmx.LBM._engine import SCEngine
+    w = WORKLOADS[name]
+    tenet = Tenet(0)
+    n0 = configs.sphere3d() if w['init'] == 'sphere3d' else configs.droplets2d()
+    mrt = hermite.mrt_tables(w['xi'], mrt_S(w['xi'])) if w['coll'] == 'mrt' else None
+    rho0 = 1.0 if w['prec'] == 'fp32s' else 0.0
+    eng = SCEngine(tenet, w['xi'], w['es'], w['dims'], 'exp', configs.SC_G, 1.0,
+                   precision=w['prec'], rho0=rho0, mrt=mrt)
+    V, D = eng.V, eng.D
+    n = IdpyMemory.OnDevice(n0, tenet)
+    u = IdpyMemory.Zeros(V * D, np.float64, tenet)
+    pop = PopArray(eng, tenet)
+    swap = PopArray(eng, tenet)
+    eng.init_pop(pop, n, u)
+    del n0
+    m0, _ = eng.mass_momentum(pop)
+    if warmup > 0 and eng.run(pop, swap, warmup):
+        pop, swap = swap, pop
+    tenet.Sync()
+    t0 = time.perf_counter()
+    if eng.run(pop, swap, steps):
+        pop, swap = swap, pop
+    tenet.Sync()
+    wall = time.perf_counter() - t0
+    ev_ms = eng.last_run_ms()
+    m1, _ = eng.mass_momentum(pop)
+    launch = eng.get_launch()
+    itemsize = 8 if w['prec'] == 'fp64' else 4
+    B_alg = 2 * eng.Q * itemsize
+    res = {
+        'V': V, 'Q': eng.Q, 'wall_s': wall, 'event_ms': ev_ms,
+        'mlups': V * steps / wall / 1e6,
+        'kernel_ms': ev_ms / steps,
+        'B_alg': B_alg,
+        'mass_rel_drift': abs(m1 - m0) / m0,
+        'launch': {'tile_x': launch[0], 'tile_y': launch[1], 'planes_per_block': launch[2]},
+        'device': tenet.GetDeviceName(),
+    }
+    eng.close()
+    tenet.End()
+    return res
+
+
+def cpu_baseline(name, budget_s=12.0):
+    """Oracle (serial C++ restatement of the reference's CTypes step, AoS
+    layout as on CTypes) on one core, on a bounded sample of the workload:
+    the same lattice kind, stencil, collision and initial profile on a
+    smaller periodic box, timed over whole steps."""
+    from idmx import configs
+    from oracle import cpuref, hermite_mrt
+    w = WORKLOADS[name]
+    dims = (64, 64, 64) if w['xi'] == 'D3Q19' else (512, 512)
+    if w['init'] == 'sphere3d':
+        n0 = configs.droplet(dims, 10.0, 2.0)
+    else:
+        n0 = configs.droplets2d(L=512, count=4, seed=configs.SEED0 + 2)
+    st = cpuref.State(w['xi'], dims, aos=True)
+    V = int(np.prod(dims))
+    tmp = cpuref.State(w['xi'], dims)
+    cpuref.init_pop(tmp, n0, np.zeros(V * len(dims)))
+    st.pop[:] = tmp.pop.reshape(st.Q, V).T.ravel()
+    mats = None
+    if w['coll'] == 'mrt':
+        import sympy as sp
+        mats = hermite_mrt.mrt_matrices(w['xi'], sp.eye(st.Q))
+    t0 = time.perf_counter()
+    cpuref.meta_run(st, w['es'], configs.SC_G, 1.0, nsteps=1, mrt=mats)
+    one = time.perf_counter() - t0
+    k = max(1, int(budget_s / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    cpuref.meta_run(st, w['es'], configs.SC_G, 1.0, nsteps=k, mrt=mats)
+    dt = time.perf_counter() - t0
+    return {'value': V * k / dt / 1e6, 'unit': 'MLUPS', 'cores': 1, 'kind': 'port',
+            'sample': '%s %s %s FP64 (AoS, as CTypes) on %s, %d steps in %.1f s'
+                      % (w['xi'], w['es'], w['coll'].upper(), 'x'.join(map(str, dims)), k, dt)}
+
+
+def traffic_from_profiles(name):
+    p = os.path.join(ROOT, 'profiles', 'traffic.json')
+    if os.path.exists(p):
+        try:
+            return json.load(open(p)).get(name)
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--workload', default='d3q19_mrt_256', choices=sorted(WORKLOADS))
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--extra', default='d2q9_4096_e8,d3q19_bgk_256,d3q19_mrt_256_fp32s',
+                    help='comma-separated extra workloads reported under "extra"')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path '
+              '(see the sharding design); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get('RANK', '0'))
+    name = args.workload
+    w = WORKLOADS[name]
+    r = run_gpu(name, args.steps, args.warmup)
+    achieved = r['B_alg'] * r['V'] / (r['kernel_ms'] / 1e3) / 1e9
+    line = {
+        'metric': 'MLUPS & %HBM roofline (D3Q19 SC)',
+        'value': round(r['mlups'], 1),
+        'unit': 'MLUPS',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(r['wall_s'] * 1e3 / args.steps, 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64' if w['prec'] == 'fp64' else 'f32',
+        'data': 'synthetic',
+        'config': workload_desc(name, w),
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
+                     'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4),
+                     'traffic': traffic_from_profiles(name),
+                     'bytes_per_site': r['B_alg'], 'kernel_ms': round(r['kernel_ms'], 4)},
+        'mass_rel_drift': r['mass_rel_drift'],
+        'launch': r['launch'],
+        'device': r['device'],
+    }
+    extra = {}
+    for x in [e for e in args.extra.split(',') if e and e != name]:
+        rx = run_gpu(x, max(20, args.steps // 4), max(5, args.warmup // 4))
+        ax = rx['B_alg'] * rx['V'] / (rx['kernel_ms'] / 1e3) / 1e9
+        extra[x] = {'mlups': round(rx['mlups'], 1), 'hbm_frac': round(ax / PEAK_HBM_GBS, 4),
+                    'achieved_GBs': round(ax, 1), 'kernel_ms': round(rx['kernel_ms'], 4),
+                    'mass_rel_drift': rx['mass_rel_drift']}
+    if extra:
+        line['extra'] = extra
+    if rank == 0 and not args.no_cpu:
+        line['cpu_baseline'] = cpu_baseline(name)
+    if rank == 0:
+        print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

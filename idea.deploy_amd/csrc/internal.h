@@ -1,0 +1,64 @@
+// internal.h -- context/simulation objects shared by the translation units of
+// libidmx.so (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/idmx.h"
+#include "sc_kernels.h"
+
+namespace idmx {
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define IDMX_HIP(call)                                        \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return ::idmx::hip_fail(e_, #call); \
+  } while (0)
+
+}  // namespace idmx
+
+struct idmx_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* partials = nullptr;      // device scratch for reductions
+  size_t partials_cap = 0;         // in doubles
+  double* host_result = nullptr;   // pinned, 16 doubles
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct SimOps;
+
+struct idmx_sc_s {
+  idmx_ctx_s* ctx = nullptr;
+  idmx_sc_desc desc{};
+  int D = 0, Q = 0;
+  idmx::Geom geo{};
+  size_t pop_elem = 8;       // bytes per population value
+  void* mrt_dev = nullptr;   // MRT tables in the working precision
+  double last_ms = 0;
+  const SimOps* ops = nullptr;
+  // fused launch shape (chosen at create time)
+  int tx = 0, ty = 0, H = 0;
+};
+
+// Per-(velocity set, E-stencil, precision, collision) entry points.
+struct SimOps {
+  int (*run)(idmx_sc_s*, void** pop, void** swap, double* n, double* psi,
+             double* u, int nsteps, int flags);
+  int (*density_psi)(idmx_sc_s*, double* n, double* psi, const void* pop);
+  int (*fcs)(idmx_sc_s*, void* swap, const void* pop, const double* n,
+             const double* psi, double* u);
+  int (*moments)(idmx_sc_s*, const void* pop, double* n, double* psi, double* u);
+  int (*density_velocity)(idmx_sc_s*, const void* pop, double* n, double* u);
+  int (*init_pop)(idmx_sc_s*, void* pop, const double* n, const double* u);
+  int (*mass_mom)(idmx_sc_s*, const void* pop, double* mass, double* mom);
+  int (*to_f64)(idmx_sc_s*, const void* pop, double* dst);
+  int (*from_f64)(idmx_sc_s*, void* pop, const double* src);
+};
+
+const SimOps* get_ops_d2q9(int estencil, int precision, int collision);
+const SimOps* get_ops_d3q19(int estencil, int precision, int collision);

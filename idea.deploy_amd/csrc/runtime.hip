@@ -1,0 +1,432 @@
+// runtime.hip -- context, device memory, reductions and the Shan-Chen
+// simulation object of libidmx.so (C ABI declared in include/idmx.h).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace idmx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return IDMX_E_HIP;
+}
+
+template <int OP>
+__global__ void __launch_bounds__(256)
+k_reduce_partial(const double* __restrict__ a, const double* __restrict__ b,
+                 int64_t n, double* __restrict__ out) {
+  // OP 0: sum a; 1: max a; 2: max |a - b|
+  double v = OP == 0 ? 0.0 : -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (OP == 0) v += a[i];
+    else if (OP == 1) v = fmax(v, a[i]);
+    else v = fmax(v, fabs(a[i] - b[i]));
+  }
+  v = block_reduce<OP == 0 ? 0 : 1>(v);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+template <int OP>
+static int reduce(idmx_ctx_s* c, const double* a, const double* b, int64_t n,
+                  double* out) {
+  if (!c || !a || !out || n < 0 || (OP == 2 && !b)) {
+    set_error("reduction: invalid argument");
+    return IDMX_E_ARG;
+  }
+  if (n == 0) {
+    *out = OP == 0 ? 0.0 : -INFINITY;
+    return IDMX_OK;
+  }
+  const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL((k_reduce_partial<OP>), dim3(grid), dim3(256), 0, c->stream, a, b, n,
+                     c->partials);
+  IDMX_HIP(hipGetLastError());
+  std::vector<double> h(grid);
+  IDMX_HIP(hipMemcpyAsync(h.data(), c->partials, sizeof(double) * grid,
+                          hipMemcpyDeviceToHost, c->stream));
+  IDMX_HIP(hipStreamSynchronize(c->stream));
+  double r = h[0];
+  for (unsigned k = 1; k < grid; k++) r = OP == 0 ? r + h[k] : std::max(r, h[k]);
+  *out = r;
+  return IDMX_OK;
+}
+
+// K_CheckUMeta + IdpyMemory.Sum/Max (LBMKernelsMeta.py, LBM.py:1500-1527,
+// MultiPhaseLoopChecks.py:96-163): per site delta = sum_d |u - old_u|,
+// old_u <- u, u2 = sum_d u^2; per block partial sum(delta) and max(u2).
+__global__ void __launch_bounds__(256)
+k_check_u(const double* __restrict__ u, double* __restrict__ old_u, int64_t V, int D,
+          double* __restrict__ out) {
+  double sd = 0.0, mu = 0.0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    double dl = 0.0, u2 = 0.0;
+    for (int d = 0; d < D; d++) {
+      const double v = u[g + d * V];
+      u2 += v * v;
+      dl += fabs(v - old_u[g + d * V]);
+      old_u[g + d * V] = v;
+    }
+    sd += dl;
+    mu = fmax(mu, u2);
+  }
+  sd = block_reduce<0>(sd);
+  mu = block_reduce<1>(mu);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = sd;
+    out[2 * blockIdx.x + 1] = mu;
+  }
+}
+
+}  // namespace idmx
+
+using namespace idmx;
+
+#define CHECK_ARG(cond, msg)          \
+  do {                                \
+    if (!(cond)) {                    \
+      ::idmx::set_error(msg);         \
+      return IDMX_E_ARG;              \
+    }                                 \
+  } while (0)
+
+extern "C" {
+
+int idmx_abi_version(void) { return IDMX_ABI_VERSION; }
+
+int idmx_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return IDMX_E_ARG;
+  snprintf(buf, len, "%s", g_err.c_str());
+  return IDMX_OK;
+}
+
+int idmx_device_count(int* count) {
+  CHECK_ARG(count, "idmx_device_count: null");
+  IDMX_HIP(hipGetDeviceCount(count));
+  return IDMX_OK;
+}
+
+int idmx_ctx_create(int device, idmx_ctx_t* out) {
+  CHECK_ARG(out, "idmx_ctx_create: null out");
+  int n = 0;
+  IDMX_HIP(hipGetDeviceCount(&n));
+  CHECK_ARG(device >= 0 && device < n, "idmx_ctx_create: no such device");
+  IDMX_HIP(hipSetDevice(device));
+  idmx_ctx_s* c = new idmx_ctx_s();
+  c->device = device;
+  hipError_t e = hipStreamCreate(&c->stream);
+  if (e == hipSuccess) {
+    c->partials_cap = 4 * 1024;
+    e = hipMalloc(&c->partials, sizeof(double) * c->partials_cap);
+  }
+  if (e == hipSuccess) e = hipHostMalloc(&c->host_result, sizeof(double) * 16, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    idmx_ctx_destroy(c);
+    return hip_fail(e, "idmx_ctx_create");
+  }
+  *out = c;
+  return IDMX_OK;
+}
+
+int idmx_ctx_destroy(idmx_ctx_t c) {
+  if (!c) return IDMX_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->partials) hipFree(c->partials);
+  if (c->host_result) hipHostFree(c->host_result);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return IDMX_OK;
+}
+
+int idmx_sync(idmx_ctx_t c) {
+  CHECK_ARG(c, "idmx_sync: null context");
+  IDMX_HIP(hipStreamSynchronize(c->stream));
+  return IDMX_OK;
+}
+
+int idmx_device_name(idmx_ctx_t c, char* buf, size_t len) {
+  CHECK_ARG(c && buf && len, "idmx_device_name: invalid argument");
+  hipDeviceProp_t p;
+  IDMX_HIP(hipGetDeviceProperties(&p, c->device));
+  snprintf(buf, len, "%s (%s)", p.name, p.gcnArchName);
+  return IDMX_OK;
+}
+
+int idmx_alloc(idmx_ctx_t c, size_t bytes, void** dptr) {
+  CHECK_ARG(c && dptr, "idmx_alloc: invalid argument");
+  IDMX_HIP(hipSetDevice(c->device));
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+  if (e == hipErrorOutOfMemory) {
+    set_error("idmx_alloc: out of device memory");
+    return IDMX_E_NOMEM;
+  }
+  IDMX_HIP(e);
+  *dptr = p;
+  return IDMX_OK;
+}
+
+int idmx_free(idmx_ctx_t c, void* p) {
+  CHECK_ARG(c, "idmx_free: null context");
+  if (!p) return IDMX_OK;
+  IDMX_HIP(hipStreamSynchronize(c->stream));
+  IDMX_HIP(hipFree(p));
+  return IDMX_OK;
+}
+
+int idmx_h2d(idmx_ctx_t c, void* dst, const void* src, size_t bytes) {
+  CHECK_ARG(c && dst && src, "idmx_h2d: invalid argument");
+  IDMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  IDMX_HIP(hipStreamSynchronize(c->stream));
+  return IDMX_OK;
+}
+
+int idmx_d2h(idmx_ctx_t c, void* dst, const void* src, size_t bytes) {
+  CHECK_ARG(c && dst && src, "idmx_d2h: invalid argument");
+  IDMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  IDMX_HIP(hipStreamSynchronize(c->stream));
+  return IDMX_OK;
+}
+
+int idmx_d2d(idmx_ctx_t c, void* dst, const void* src, size_t bytes) {
+  CHECK_ARG(c && dst && src, "idmx_d2d: invalid argument");
+  IDMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return IDMX_OK;
+}
+
+int idmx_memset(idmx_ctx_t c, void* dst, int value, size_t bytes) {
+  CHECK_ARG(c && dst, "idmx_memset: invalid argument");
+  IDMX_HIP(hipMemsetAsync(dst, value, bytes, c->stream));
+  return IDMX_OK;
+}
+
+int idmx_sum_f64(idmx_ctx_t c, const double* x, int64_t n, double* out) {
+  return reduce<0>(c, x, nullptr, n, out);
+}
+int idmx_max_f64(idmx_ctx_t c, const double* x, int64_t n, double* out) {
+  return reduce<1>(c, x, nullptr, n, out);
+}
+int idmx_max_abs_diff_f64(idmx_ctx_t c, const double* a, const double* b, int64_t n,
+                          double* out) {
+  return reduce<2>(c, a, b, n, out);
+}
+
+int idmx_check_u(idmx_ctx_t c, const double* u, double* old_u, int64_t V, int D,
+                 double* sum_delta, double* max_u2) {
+  CHECK_ARG(c && u && old_u && sum_delta && max_u2 && V > 0 && D >= 1 && D <= 3,
+            "idmx_check_u: invalid argument");
+  const unsigned grid = (unsigned)std::min<int64_t>((V + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_check_u, dim3(grid), dim3(256), 0, c->stream, u, old_u, V, D,
+                     c->partials);
+  IDMX_HIP(hipGetLastError());
+  std::vector<double> h(2 * grid);
+  IDMX_HIP(hipMemcpyAsync(h.data(), c->partials, sizeof(double) * 2 * grid,
+                          hipMemcpyDeviceToHost, c->stream));
+  IDMX_HIP(hipStreamSynchronize(c->stream));
+  double sd = 0.0, mu = 0.0;
+  for (unsigned k = 0; k < grid; k++) {
+    sd += h[2 * k];
+    mu = std::max(mu, h[2 * k + 1]);
+  }
+  *sum_delta = sd;
+  *max_u2 = mu;
+  return IDMX_OK;
+}
+
+int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
+  CHECK_ARG(c && d && out, "idmx_sc_create: invalid argument");
+  CHECK_ARG(d->xi == IDMX_D2Q9 || d->xi == IDMX_D3Q19, "idmx_sc_create: unknown velocity set");
+  CHECK_ARG(d->estencil == IDMX_E4 || d->estencil == IDMX_E6 || d->estencil == IDMX_E8,
+            "idmx_sc_create: unknown E-stencil");
+  CHECK_ARG(d->psi_kind >= 0 && d->psi_kind <= 2, "idmx_sc_create: unknown psi kind");
+  CHECK_ARG(d->collision == IDMX_BGK || d->collision == IDMX_MRT,
+            "idmx_sc_create: unknown collision");
+  CHECK_ARG(d->precision == IDMX_PREC_F64 || d->precision == IDMX_PREC_F32S,
+            "idmx_sc_create: unknown precision");
+  CHECK_ARG(d->tau > 0, "idmx_sc_create: tau must be positive");
+  const int D = d->xi == IDMX_D2Q9 ? 2 : 3;
+  const int Q = d->xi == IDMX_D2Q9 ? 9 : 19;
+  const int R = d->estencil == IDMX_E4 ? 1 : 2;
+  for (int k = 0; k < D; k++)
+    CHECK_ARG(d->dims[k] >= 2 * R + 1 && d->dims[k] <= (1 << 20),
+              "idmx_sc_create: each lattice size must be >= 2*reach+1 and <= 2^20");
+  if (D == 2) CHECK_ARG(d->dims[2] <= 1, "idmx_sc_create: dims[2] must be 1 for D2Q9");
+  const int64_t V = (int64_t)d->dims[0] * d->dims[1] * (D == 3 ? d->dims[2] : 1);
+  CHECK_ARG(V * Q < ((int64_t)1 << 40), "idmx_sc_create: lattice too large");
+  if (d->collision == IDMX_MRT)
+    CHECK_ARG(d->mrt_A && d->mrt_beta && d->mrt_gamma, "idmx_sc_create: MRT tables missing");
+
+  idmx_sc_s* s = new idmx_sc_s();
+  s->ctx = c;
+  s->desc = *d;
+  s->desc.mrt_A = s->desc.mrt_beta = s->desc.mrt_gamma = nullptr;
+  if (D == 2) s->desc.dims[2] = 1;
+  if (d->precision == IDMX_PREC_F64) s->desc.rho0 = 0.0;
+  s->D = D;
+  s->Q = Q;
+  s->geo.L0 = d->dims[0];
+  s->geo.L1 = d->dims[1];
+  s->geo.L2 = D == 3 ? d->dims[2] : 1;
+  s->geo.V = V;
+  s->pop_elem = d->precision == IDMX_PREC_F64 ? 8 : 4;
+  s->ops = D == 2 ? get_ops_d2q9(d->estencil, d->precision, d->collision)
+                  : get_ops_d3q19(d->estencil, d->precision, d->collision);
+  if (!s->ops) {
+    delete s;
+    set_error("idmx_sc_create: unsupported combination");
+    return IDMX_E_UNSUPPORTED;
+  }
+  // Fused launch shape: rows of 256 (or 64) in 2D, 32 x 8 tiles in 3D; march
+  // segments sized to give >= 2048 blocks where the lattice allows it.
+  if (D == 2) {
+    s->tx = d->dims[0] >= 256 ? 256 : 64;
+    s->ty = 1;
+  } else {
+    s->tx = 32;
+    s->ty = 8;
+  }
+  {
+    const int ntx = (s->geo.L0 + s->tx - 1) / s->tx;
+    const int nty = D == 3 ? (s->geo.L1 + s->ty - 1) / s->ty : 1;
+    const int Lm = D == 3 ? s->geo.L2 : s->geo.L1;
+    const int want_seg = std::max(1, 2048 / std::max(1, ntx * nty));
+    int H = (Lm + want_seg - 1) / want_seg;
+    H = std::max(H, 8);
+    s->H = std::min(H, Lm);
+  }
+  if (d->collision == IDMX_MRT) {
+    const int NB = D == 2 ? 6 : 10;
+    const size_t cnt = (size_t)Q * Q + 2 * (size_t)NB * Q;
+    std::vector<double> hd(cnt);
+    memcpy(hd.data(), d->mrt_A, sizeof(double) * Q * Q);
+    memcpy(hd.data() + Q * Q, d->mrt_beta, sizeof(double) * NB * Q);
+    memcpy(hd.data() + Q * Q + NB * Q, d->mrt_gamma, sizeof(double) * NB * Q);
+    hipError_t e = hipMalloc(&s->mrt_dev, cnt * s->pop_elem);
+    if (e == hipSuccess) {
+      if (s->pop_elem == 8) {
+        e = hipMemcpy(s->mrt_dev, hd.data(), cnt * 8, hipMemcpyHostToDevice);
+      } else {
+        std::vector<float> hf(hd.begin(), hd.end());
+        e = hipMemcpy(s->mrt_dev, hf.data(), cnt * 4, hipMemcpyHostToDevice);
+      }
+    }
+    if (e != hipSuccess) {
+      if (s->mrt_dev) hipFree(s->mrt_dev);
+      delete s;
+      return hip_fail(e, "idmx_sc_create: MRT tables");
+    }
+  }
+  *out = s;
+  return IDMX_OK;
+}
+
+int idmx_sc_destroy(idmx_sc_t s) {
+  if (!s) return IDMX_OK;
+  if (s->mrt_dev) {
+    hipStreamSynchronize(s->ctx->stream);
+    hipFree(s->mrt_dev);
+  }
+  delete s;
+  return IDMX_OK;
+}
+
+int idmx_sc_pop_bytes(idmx_sc_t s, size_t* bytes) {
+  CHECK_ARG(s && bytes, "idmx_sc_pop_bytes: invalid argument");
+  *bytes = (size_t)s->Q * (size_t)s->geo.V * s->pop_elem;
+  return IDMX_OK;
+}
+
+int idmx_sc_set_launch(idmx_sc_t s, int tx, int H) {
+  CHECK_ARG(s, "idmx_sc_set_launch: null simulation");
+  if (s->D == 2) CHECK_ARG(tx == 64 || tx == 256, "idmx_sc_set_launch: 2D tile width is 64 or 256");
+  else CHECK_ARG(tx == 32 || tx == 64, "idmx_sc_set_launch: 3D tile width is 32 (32x8) or 64 (64x4)");
+  CHECK_ARG(H >= 1, "idmx_sc_set_launch: H must be >= 1");
+  s->tx = tx;
+  s->ty = s->D == 2 ? 1 : (tx == 64 ? 4 : 8);
+  s->H = H;
+  return IDMX_OK;
+}
+
+int idmx_sc_get_launch(idmx_sc_t s, int* tx, int* ty, int* H) {
+  CHECK_ARG(s && tx && ty && H, "idmx_sc_get_launch: invalid argument");
+  *tx = s->tx;
+  *ty = s->ty;
+  *H = s->H;
+  return IDMX_OK;
+}
+
+int idmx_sc_init_pop(idmx_sc_t s, void* pop, const double* n, const double* u) {
+  CHECK_ARG(s && pop && n && u, "idmx_sc_init_pop: invalid argument");
+  return s->ops->init_pop(s, pop, n, u);
+}
+
+int idmx_sc_density_psi(idmx_sc_t s, double* n, double* psi, const void* pop) {
+  CHECK_ARG(s && n && psi && pop, "idmx_sc_density_psi: invalid argument");
+  return s->ops->density_psi(s, n, psi, pop);
+}
+
+int idmx_sc_force_collide_stream(idmx_sc_t s, void* swap, const void* pop,
+                                 const double* n, const double* psi) {
+  CHECK_ARG(s && swap && pop && n && psi && swap != pop,
+            "idmx_sc_force_collide_stream: invalid argument");
+  return s->ops->fcs(s, swap, pop, n, psi, nullptr);
+}
+
+int idmx_sc_run(idmx_sc_t s, void** pop, void** swap, double* n, double* psi, double* u,
+                int nsteps, int flags) {
+  CHECK_ARG(s && pop && swap && *pop && *swap && *pop != *swap,
+            "idmx_sc_run: invalid population buffers");
+  CHECK_ARG(nsteps >= 0, "idmx_sc_run: nsteps must be >= 0");
+  if (nsteps == 0) {
+    s->last_ms = 0;
+    return IDMX_OK;
+  }
+  return s->ops->run(s, pop, swap, n, psi, u, nsteps, flags);
+}
+
+int idmx_sc_moments(idmx_sc_t s, const void* pop, double* n, double* psi, double* u) {
+  CHECK_ARG(s && pop && n && psi && u, "idmx_sc_moments: invalid argument");
+  return s->ops->moments(s, pop, n, psi, u);
+}
+
+int idmx_sc_density_velocity(idmx_sc_t s, const void* pop, double* n, double* u) {
+  CHECK_ARG(s && pop && n && u, "idmx_sc_density_velocity: invalid argument");
+  return s->ops->density_velocity(s, pop, n, u);
+}
+
+int idmx_sc_mass_momentum(idmx_sc_t s, const void* pop, double* mass, double* mom) {
+  CHECK_ARG(s && pop && mass && mom, "idmx_sc_mass_momentum: invalid argument");
+  return s->ops->mass_mom(s, pop, mass, mom);
+}
+
+int idmx_sc_pop_to_f64(idmx_sc_t s, const void* pop, double* dst) {
+  CHECK_ARG(s && pop && dst, "idmx_sc_pop_to_f64: invalid argument");
+  return s->ops->to_f64(s, pop, dst);
+}
+
+int idmx_sc_pop_from_f64(idmx_sc_t s, void* pop, const double* src) {
+  CHECK_ARG(s && pop && src, "idmx_sc_pop_from_f64: invalid argument");
+  return s->ops->from_f64(s, pop, src);
+}
+
+int idmx_sc_last_run_ms(idmx_sc_t s, double* ms) {
+  CHECK_ARG(s && ms, "idmx_sc_last_run_ms: invalid argument");
+  *ms = s->last_ms;
+  return IDMX_OK;
+}
+
+}  // extern "C"

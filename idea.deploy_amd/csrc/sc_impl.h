@@ -1,0 +1,211 @@
+// sc_impl.h -- host-side launch logic for one (velocity set, E-stencil,
+// precision, collision) combination. Included by sc_d2q9.hip / sc_d3q19.hip,
+// which instantiate the combinations of their velocity set.
+#pragma once
+#include <algorithm>
+
+#include "internal.h"
+
+namespace idmx {
+
+inline unsigned grid_for(int64_t V) {
+  const int64_t b = (V + 255) / 256;
+  return (unsigned)std::min<int64_t>(b, 65536);
+}
+
+template <class S, class ES, class T, bool SHIFT, int COLL>
+struct Impl {
+  static Phys<T> phys(const idmx_sc_s* s) {
+    Phys<T> P;
+    P.G = (T)s->desc.SC_G;
+    P.omega = (T)(1.0 / s->desc.tau);
+    P.rho0 = SHIFT ? (T)s->desc.rho0 : T(0);
+    P.psi_kind = s->desc.psi_kind;
+    const T* m = (const T*)s->mrt_dev;
+    constexpr int NB = NBasis<S::D>::value;
+    P.mrtA = m;
+    P.mrtB = m ? m + S::Q * S::Q : nullptr;
+    P.mrtG = m ? m + S::Q * S::Q + NB * S::Q : nullptr;
+    return P;
+  }
+
+  template <int TX, int TY, bool WF>
+  static hipError_t launch_fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
+                                 double* psi, double* u) {
+    const Geom& g = s->geo;
+    const int ntx = (g.L0 + TX - 1) / TX;
+    const int nty = S::D == 3 ? (g.L1 + TY - 1) / TY : 1;
+    const int Lm = S::D == 3 ? g.L2 : g.L1;
+    const int H = std::min(s->H, Lm);
+    const int nseg = (Lm + H - 1) / H;
+    const int ntiles = ntx * nty * nseg;
+    const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
+    hipLaunchKernelGGL((k_fused<S, ES, T, SHIFT, COLL, WF, TX, TY>), dim3(grid),
+                       dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H,
+                       ntx, nty, ntiles, n, psi, u);
+    return hipGetLastError();
+  }
+
+  template <bool WF>
+  static hipError_t fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
+                          double* psi, double* u) {
+    if constexpr (S::D == 2) {
+      if (s->tx == 256) return launch_fused<256, 1, WF>(s, pop, swap, n, psi, u);
+      return launch_fused<64, 1, WF>(s, pop, swap, n, psi, u);
+    } else {
+      if (s->tx == 64) return launch_fused<64, 4, WF>(s, pop, swap, n, psi, u);
+      return launch_fused<32, 8, WF>(s, pop, swap, n, psi, u);
+    }
+  }
+
+  static hipError_t two_pass(idmx_sc_s* s, const T* pop, T* swap, double* n,
+                             double* psi, double* u, bool wf) {
+    const unsigned grid = grid_for(s->geo.V);
+    hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid), dim3(256), 0,
+                       s->ctx->stream, pop, s->geo, phys(s), n, psi);
+    if (wf)
+      hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, true>), dim3(grid), dim3(256), 0,
+                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u);
+    else
+      hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, false>), dim3(grid), dim3(256), 0,
+                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u);
+    return hipGetLastError();
+  }
+
+  static int run(idmx_sc_s* s, void** pop, void** swap, double* n, double* psi,
+                 double* u, int nsteps, int flags) {
+    const bool wf = (flags & IDMX_RUN_WRITE_FIELDS) != 0;
+    const bool twop = (flags & IDMX_RUN_TWO_PASS) != 0;
+    if (twop && (!n || !psi)) {
+      set_error("idmx_sc_run: the two-pass step needs n and psi buffers");
+      return IDMX_E_ARG;
+    }
+    if (wf && (!n || !psi || !u)) {
+      set_error("idmx_sc_run: IDMX_RUN_WRITE_FIELDS needs n, psi and u buffers");
+      return IDMX_E_ARG;
+    }
+    hipStream_t st = s->ctx->stream;
+    IDMX_HIP(hipEventRecord(s->ctx->ev0, st));
+    for (int k = 0; k < nsteps; k++) {
+      const bool last = k == nsteps - 1;
+      const T* a = (const T*)*pop;
+      T* b = (T*)*swap;
+      hipError_t e;
+      if (twop) e = two_pass(s, a, b, n, psi, u, wf && last);
+      else if (wf && last) e = fused<true>(s, a, b, n, psi, u);
+      else e = fused<false>(s, a, b, nullptr, nullptr, nullptr);
+      if (e != hipSuccess) return hip_fail(e, "idmx_sc_run launch");
+      std::swap(*pop, *swap);
+    }
+    IDMX_HIP(hipEventRecord(s->ctx->ev1, st));
+    IDMX_HIP(hipEventSynchronize(s->ctx->ev1));
+    float ms = 0;
+    IDMX_HIP(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
+    s->last_ms = ms;
+    return IDMX_OK;
+  }
+
+  static int density_psi(idmx_sc_s* s, double* n, double* psi, const void* pop) {
+    hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid_for(s->geo.V)), dim3(256),
+                       0, s->ctx->stream, (const T*)pop, s->geo, phys(s), n, psi);
+    IDMX_HIP(hipGetLastError());
+    return IDMX_OK;
+  }
+
+  static int fcs(idmx_sc_s* s, void* swap, const void* pop, const double* n,
+                 const double* psi, double* u) {
+    const unsigned grid = grid_for(s->geo.V);
+    if (u)
+      hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, true>), dim3(grid), dim3(256), 0,
+                         s->ctx->stream, (const T*)pop, (T*)swap, n, psi, s->geo, phys(s), u);
+    else
+      hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, false>), dim3(grid), dim3(256), 0,
+                         s->ctx->stream, (const T*)pop, (T*)swap, n, psi, s->geo, phys(s),
+                         nullptr);
+    IDMX_HIP(hipGetLastError());
+    return IDMX_OK;
+  }
+
+  static int moments(idmx_sc_s* s, const void* pop, double* n, double* psi, double* u) {
+    const unsigned grid = grid_for(s->geo.V);
+    hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid), dim3(256), 0,
+                       s->ctx->stream, (const T*)pop, s->geo, phys(s), n, psi);
+    hipLaunchKernelGGL((k_velocity<S, ES, T, SHIFT>), dim3(grid), dim3(256), 0,
+                       s->ctx->stream, (const T*)pop, n, psi, s->geo, s->desc.SC_G, u);
+    IDMX_HIP(hipGetLastError());
+    return IDMX_OK;
+  }
+
+  // K_ComputeMoments (legacy, LBM.py:1667-1698): n and u = sum xi f / n.
+  static int density_velocity(idmx_sc_s* s, const void* pop, double* n, double* u) {
+    const unsigned grid = grid_for(s->geo.V);
+    hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid), dim3(256), 0,
+                       s->ctx->stream, (const T*)pop, s->geo, phys(s), n, nullptr);
+    // G = 0: no force term; n stands in for the (unused) psi argument
+    hipLaunchKernelGGL((k_velocity<S, ES, T, SHIFT>), dim3(grid), dim3(256), 0,
+                       s->ctx->stream, (const T*)pop, n, n, s->geo, 0.0, u);
+    IDMX_HIP(hipGetLastError());
+    return IDMX_OK;
+  }
+
+  static int init_pop(idmx_sc_s* s, void* pop, const double* n, const double* u) {
+    hipLaunchKernelGGL((k_init_pop<S, T, SHIFT>), dim3(grid_for(s->geo.V)), dim3(256), 0,
+                       s->ctx->stream, (T*)pop, n, u, s->geo, s->desc.rho0);
+    IDMX_HIP(hipGetLastError());
+    return IDMX_OK;
+  }
+
+  static int mass_mom(idmx_sc_s* s, const void* pop, double* mass, double* mom) {
+    const unsigned grid = std::min<unsigned>(grid_for(s->geo.V), 1024);
+    idmx_ctx_s* c = s->ctx;
+    if ((size_t)grid * 4 > c->partials_cap) {
+      set_error("idmx_sc_mass_momentum: reduction scratch too small");
+      return IDMX_E_NOMEM;
+    }
+    hipLaunchKernelGGL((k_mass_mom_partial<S, T, SHIFT>), dim3(grid), dim3(256), 0,
+                       c->stream, (const T*)pop, s->geo, s->desc.rho0, c->partials);
+    IDMX_HIP(hipGetLastError());
+    static thread_local double host[1024 * 4];
+    IDMX_HIP(hipMemcpyAsync(host, c->partials, sizeof(double) * 4 * grid,
+                            hipMemcpyDeviceToHost, c->stream));
+    IDMX_HIP(hipStreamSynchronize(c->stream));
+    double acc[4] = {0, 0, 0, 0};
+    for (unsigned b = 0; b < grid; b++)
+      for (int k = 0; k < 4; k++) acc[k] += host[b * 4 + k];
+    *mass = acc[0];
+    for (int d = 0; d < S::D; d++) mom[d] = acc[1 + d];
+    return IDMX_OK;
+  }
+
+  static int to_f64(idmx_sc_s* s, const void* pop, double* dst) {
+    hipLaunchKernelGGL((k_pop_to_f64<S, T, SHIFT>), dim3(grid_for(s->geo.V)), dim3(256), 0,
+                       s->ctx->stream, (const T*)pop, dst, s->geo, s->desc.rho0);
+    IDMX_HIP(hipGetLastError());
+    return IDMX_OK;
+  }
+
+  static int from_f64(idmx_sc_s* s, void* pop, const double* src) {
+    hipLaunchKernelGGL((k_pop_from_f64<S, T, SHIFT>), dim3(grid_for(s->geo.V)), dim3(256), 0,
+                       s->ctx->stream, (T*)pop, src, s->geo, s->desc.rho0);
+    IDMX_HIP(hipGetLastError());
+    return IDMX_OK;
+  }
+
+  static const SimOps* ops() {
+    static const SimOps o = {run, density_psi, fcs, moments, density_velocity, init_pop,
+                             mass_mom, to_f64, from_f64};
+    return &o;
+  }
+};
+
+// Table over precision and collision for one (S, ES).
+template <class S, class ES>
+const SimOps* ops_for(int precision, int collision) {
+  if (precision == IDMX_PREC_F64)
+    return collision == IDMX_BGK ? Impl<S, ES, double, false, 0>::ops()
+                                 : Impl<S, ES, double, false, 1>::ops();
+  return collision == IDMX_BGK ? Impl<S, ES, float, true, 0>::ops()
+                               : Impl<S, ES, float, true, 1>::ops();
+}
+
+}  // namespace idmx

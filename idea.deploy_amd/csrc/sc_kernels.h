@@ -1,0 +1,545 @@
+// sc_kernels.h -- HIP kernels of the Shan-Chen multiphase LBM step for gfx950.
+//
+// One time step (idpy/LBM/MultiPhase.py:1015-1097):
+//   n = sum_q f_q, psi = psi(n)                 K_ComputeDensityPsiMeta
+//   F = -G psi(x) sum_e w_e e psi(x+e)          IDShanChenForce.py:85-114
+//   u = sum_q xi_q f_q / n + F / (2n)           VelocityCode + Guo shift
+//   f*_q = BGK or MRT collision with Guo forcing (Collisions.py:163-185,
+//          96-160; Forcings.py:104-181)
+//   pop_swap[x + xi_q, q] = f*_q                push streaming
+//
+// Storage: SoA pop[q*V + g], g = x + L0*(y + L1*z). T = double, or float with
+// the shift f~_q = f_q - w_q*rho0 (SHIFT): every linear step of the update
+// commutes with the shift, so only the density needs rho0 added back.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice_tables.h"
+
+namespace idmx {
+
+// 1/c2 and 1/c2^2 for c2 = 1/3 evaluated in double (CM2, CM4 of
+// idpy/LBM/LBM.py:542-569): exactly 3 and 9.
+constexpr double kCM2 = 3.0;
+constexpr double kCM4 = 9.0;
+
+struct Geom {
+  int L0, L1, L2;  // L2 = 1 for two dimensions
+  int64_t V;
+};
+
+// Number of equilibrium / forcing basis vectors w∘{1, xi_a, xi_a xi_b}.
+template <int D> struct NBasis { static constexpr int value = D == 2 ? 6 : 10; };
+
+template <class T>
+struct Phys {
+  T G;       // Shan-Chen coupling SC_G
+  T omega;   // 1/tau
+  T rho0;    // population shift (0 unless SHIFT)
+  int psi_kind;
+  const T* mrtA;  // Q x Q     M^-1 (1-S) M
+  const T* mrtB;  // NB x Q    M^-1 S M v_k
+  const T* mrtG;  // NB x Q    M'^-1 (1-S/2) M' v_k
+};
+
+__device__ inline double dexp(double v) { return exp(v); }
+__device__ inline float dexp(float v) { return expf(v); }
+
+// psi(n) for the supported pseudopotentials (psi_code kwarg).
+template <class T>
+__device__ inline T psi_of(int kind, T n) {
+  if (kind == 0) return T(1) - dexp(-n);
+  if (kind == 1) return dexp(T(-1) / n);
+  return n;
+}
+
+// Periodic wrap of a coordinate that is at most one period out of range.
+__device__ inline int wrap(int v, int L) {
+  v = v < 0 ? v + L : v;
+  return v >= L ? v - L : v;
+}
+
+// --------------------------------------------------------------------------
+// Collision of one site. f holds the (shifted when SHIFT) populations, dn is
+// sum_q f_q of those values, (gx, gy, gz) the isotropic gradient of psi and
+// lpsi = psi(x). Returns f* in fo and the Guo-shifted velocity in u.
+template <class S, class T, bool SHIFT, int COLL>
+__device__ inline void collide(const T (&f)[S::Q], T dn, T gx, T gy, T gz,
+                               T lpsi, const Phys<T>& P, T (&fo)[S::Q],
+                               T (&u)[3]) {
+  constexpr int Q = S::Q;
+  T jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    if (S::cx[q] == 1) jx += f[q];
+    if (S::cx[q] == -1) jx -= f[q];
+    if (S::cy[q] == 1) jy += f[q];
+    if (S::cy[q] == -1) jy -= f[q];
+    if (S::cz[q] == 1) jz += f[q];
+    if (S::cz[q] == -1) jz -= f[q];
+  }
+  const T n = SHIFT ? P.rho0 + dn : dn;
+  const T mGpsi = -P.G * lpsi;
+  const T Fx = mGpsi * gx, Fy = mGpsi * gy, Fz = S::D == 3 ? mGpsi * gz : T(0);
+  const T inv_n = T(1) / n;
+  const T ux = (jx + T(0.5) * Fx) * inv_n;
+  const T uy = (jy + T(0.5) * Fy) * inv_n;
+  const T uz = S::D == 3 ? (jz + T(0.5) * Fz) * inv_n : T(0);
+  u[0] = ux;
+  u[1] = uy;
+  u[2] = uz;
+  const T uu = ux * ux + uy * uy + uz * uz;
+  const T Fu = Fx * ux + Fy * uy + Fz * uz;
+  if (COLL == 0) {
+    const T om = P.omega;
+    const T c_f = T(1) - om;
+    const T c_g = T(1) - T(0.5) * om;
+    // n-part of f_eq without the shift: SHIFT subtracts w*rho0 from f_eq,
+    // i.e. uses dn in place of n for the constant term.
+    const T n0 = SHIFT ? dn : n;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const T xu = S::cx[q] * ux + S::cy[q] * uy + S::cz[q] * uz;
+      const T xF = S::cx[q] * Fx + S::cy[q] * Fy + S::cz[q] * Fz;
+      const T w = T(S::w[q]);
+      const T feq = w * (n0 + n * (T(kCM2) * xu + T(0.5 * kCM4) * xu * xu - T(0.5 * kCM2) * uu));
+      const T fg = w * (T(kCM2) * xF + T(kCM4) * xF * xu - T(kCM2) * Fu);
+      fo[q] = c_f * f[q] + om * feq + c_g * fg;
+    }
+  } else {
+    // f* = A f + sum_k c_k beta_k + sum_k d_k gamma_k, with c_k / d_k the
+    // coordinates of f_eq / raw Guo term on w∘{1, xi_a, xi_a xi_b}.
+    constexpr int NB = NBasis<S::D>::value;
+    T c[NB], d[NB];
+    const T n0 = SHIFT ? dn : n;
+    c[0] = n0 - n * T(0.5 * kCM2) * uu;
+    d[0] = -T(kCM2) * Fu;
+    if (S::D == 2) {
+      c[1] = n * T(kCM2) * ux;  c[2] = n * T(kCM2) * uy;
+      c[3] = n * T(0.5 * kCM4) * ux * ux;
+      c[4] = n * T(0.5 * kCM4) * uy * uy;
+      c[5] = n * T(kCM4) * ux * uy;
+      d[1] = T(kCM2) * Fx;  d[2] = T(kCM2) * Fy;
+      d[3] = T(kCM4) * Fx * ux;
+      d[4] = T(kCM4) * Fy * uy;
+      d[5] = T(kCM4) * (Fx * uy + Fy * ux);
+    } else {
+      c[1] = n * T(kCM2) * ux;  c[2] = n * T(kCM2) * uy;  c[3] = n * T(kCM2) * uz;
+      c[4] = n * T(0.5 * kCM4) * ux * ux;
+      c[5] = n * T(0.5 * kCM4) * uy * uy;
+      c[6] = n * T(0.5 * kCM4) * uz * uz;
+      c[7] = n * T(kCM4) * ux * uy;
+      c[8] = n * T(kCM4) * ux * uz;
+      c[9] = n * T(kCM4) * uy * uz;
+      d[1] = T(kCM2) * Fx;  d[2] = T(kCM2) * Fy;  d[3] = T(kCM2) * Fz;
+      d[4] = T(kCM4) * Fx * ux;
+      d[5] = T(kCM4) * Fy * uy;
+      d[6] = T(kCM4) * Fz * uz;
+      d[7] = T(kCM4) * (Fx * uy + Fy * ux);
+      d[8] = T(kCM4) * (Fx * uz + Fz * ux);
+      d[9] = T(kCM4) * (Fy * uz + Fz * uy);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      T a = 0;
+#pragma unroll
+      for (int k = 0; k < Q; k++) a += P.mrtA[q * Q + k] * f[k];
+#pragma unroll
+      for (int k = 0; k < NB; k++) a += P.mrtB[k * Q + q] * c[k] + P.mrtG[k * Q + q] * d[k];
+      fo[q] = a;
+    }
+  }
+}
+
+// Lexicographic index of (x, y, z).
+__device__ inline int64_t lex(const Geom& g, int x, int y, int z) {
+  return (int64_t)x + (int64_t)g.L0 * ((int64_t)y + (int64_t)g.L1 * z);
+}
+
+// Push-stream one site's f* to its neighbours (periodic).
+template <class S, class T>
+__device__ inline void push_store(T* __restrict__ dst, const Geom& geo, int x,
+                                  int y, int z, const T (&fo)[S::Q]) {
+#pragma unroll
+  for (int q = 0; q < S::Q; q++) {
+    const int xt = S::cx[q] ? wrap(x + S::cx[q], geo.L0) : x;
+    const int yt = S::cy[q] ? wrap(y + S::cy[q], geo.L1) : y;
+    const int zt = S::cz[q] ? wrap(z + S::cz[q], geo.L2) : z;
+    __builtin_nontemporal_store(fo[q], dst + (size_t)q * geo.V + lex(geo, xt, yt, zt));
+  }
+}
+
+// --------------------------------------------------------------------------
+// Fused single-pass step.
+//
+// A block owns a TX x TY tile of the plane orthogonal to the slowest axis
+// (z in 3D; in 2D the "plane" is an x-row of TX sites and TY = 1) and marches
+// H planes along that axis. For every plane it reads ahead ("lead plane", R
+// planes ahead of the one it collides) the populations of the tile plus an
+// R-wide halo, sums them to n and stores psi(n) in an LDS ring of 2R+2
+// planes. The tile's own populations stay in registers for R iterations
+// until their plane is collided, so each population is read from HBM once
+// per step except for the halo sites and the 2R extra planes per segment.
+// One barrier per plane: the ring has one slot more than the stencil reach
+// needs, so a plane's slot is rewritten only after every thread has passed
+// the next barrier.
+//
+// Blocks are dispatched round-robin over the 8 XCDs; the tile order is
+// remapped so that each XCD walks a contiguous range of tiles (neighbouring
+// tiles share halo rows, which then hit the same L2).
+template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY)
+k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
+        Phys<T> P, int H, int ntx, int nty, int ntiles,
+        double* __restrict__ n_out, double* __restrict__ psi_out,
+        double* __restrict__ u_out) {
+  constexpr int D = S::D, Q = S::Q, R = ES::R;
+  constexpr int NT = TX * TY;
+  constexpr int PX = TX + 2 * R;
+  constexpr int PY = D == 3 ? TY + 2 * R : 1;
+  constexpr int NS = 2 * R + 2;
+  constexpr int NH = PX * PY - NT;  // halo sites per plane
+  static_assert(D == 3 || TY == 1, "2D tiles are rows");
+  constexpr int NHT = (NH + NT - 1) / NT;  // halo sites per thread
+  __shared__ T lds[NS][PY][PX];
+
+  // XCD-aware remap of the (padded to a multiple of 8) grid.
+  const int per = gridDim.x >> 3;
+  const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (tile >= ntiles) return;
+
+  int t = tile;
+  const int txi = t % ntx;
+  t /= ntx;
+  const int tyi = D == 3 ? t % nty : 0;
+  const int seg = D == 3 ? t / nty : t;
+  const int x0 = txi * TX, y0 = tyi * TY;
+  const int Lm = D == 3 ? geo.L2 : geo.L1;          // marching axis length
+  const int64_t PS = D == 3 ? (int64_t)geo.L0 * geo.L1 : (int64_t)geo.L0;
+  const int m0 = seg * H;
+  const int Hs = min(H, Lm - m0);
+
+  const int tid = threadIdx.x;
+  const int ix = tid % TX, iy = tid / TX;
+  const int x = x0 + ix, y = D == 3 ? y0 + iy : 0;
+  const bool own_valid = x < geo.L0 && (D == 2 || y < geo.L1);
+  const int xw = x % geo.L0, yw = D == 3 ? y % geo.L1 : 0;
+  const int64_t own_off = (int64_t)xw + (int64_t)geo.L0 * yw;
+
+  // This thread's halo sites: position in the haloed plane and in-plane
+  // offset. Halo site h is enumerated as: R top rows, R bottom rows, then
+  // the R left / R right columns of the TY middle rows (2D: R left, R right).
+  int hxs[NHT], hys[NHT];
+  int64_t hoff[NHT];
+#pragma unroll
+  for (int k = 0; k < NHT; k++) {
+    const int h = tid + k * NT;
+    int hx = 0, hy = 0;
+    if (D == 2) {
+      hx = h < R ? h : TX + h;
+    } else {
+      const int top = R * PX;
+      if (h < top) {
+        hy = h / PX; hx = h % PX;
+      } else if (h < 2 * top) {
+        hy = R + TY + (h - top) / PX; hx = (h - top) % PX;
+      } else {
+        const int kk = h - 2 * top;
+        hy = R + kk / (2 * R);
+        const int sd = kk % (2 * R);
+        hx = sd < R ? sd : TX + sd;
+      }
+    }
+    hxs[k] = hx;
+    hys[k] = hy;
+    const int gx = ((x0 - R + hx) % geo.L0 + geo.L0) % geo.L0;
+    const int gy = D == 3 ? ((y0 - R + hy) % geo.L1 + geo.L1) % geo.L1 : 0;
+    hoff[k] = (int64_t)gx + (int64_t)geo.L0 * gy;
+  }
+
+  T fh[R + 1][Q];  // own populations of the last R+1 lead planes
+  T dnh[R + 1];    // their sums
+
+  for (int j = 0; j < Hs + 2 * R; j++) {
+    const int mj = ((m0 - R + j) % Lm + Lm) % Lm;
+    const int64_t base = (int64_t)mj * PS;
+    const int slot = j % NS;
+    {
+      T s = 0;
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        fh[R][q] = pop[(size_t)q * geo.V + base + own_off];
+      }
+#pragma unroll
+      for (int q = 0; q < Q; q++) s += fh[R][q];
+      dnh[R] = s;
+      lds[slot][D == 3 ? R + iy : 0][R + ix] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+    }
+#pragma unroll
+    for (int k = 0; k < NHT; k++) {
+      if (tid + k * NT < NH) {
+        T s = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) s += pop[(size_t)q * geo.V + base + hoff[k]];
+        lds[slot][hys[k]][hxs[k]] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+      }
+    }
+    __syncthreads();
+    if (j >= 2 * R && own_valid) {
+      // plane with lead index j - R
+      int sl[2 * R + 1];
+#pragma unroll
+      for (int k = 0; k <= 2 * R; k++) sl[k] = (j - 2 * R + k) % NS;
+      const int cy = D == 3 ? R + iy : 0;
+      const int cx = R + ix;
+      T gx = 0, gy = 0, gz = 0;
+#pragma unroll
+      for (int e = 0; e < ES::QE; e++) {
+        const int em = D == 3 ? ES::ez[e] : ES::ey[e];
+        const int ey = D == 3 ? ES::ey[e] : 0;
+        const T v = lds[sl[R + em]][cy + ey][cx + ES::ex[e]];
+        const T w = T(ES::w[e]);
+        if (ES::ex[e] != 0) gx += T(ES::ex[e]) * w * v;
+        if (ES::ey[e] != 0) gy += T(ES::ey[e]) * w * v;
+        if (ES::ez[e] != 0) gz += T(ES::ez[e]) * w * v;
+      }
+      const T lpsi = lds[sl[R]][cy][cx];
+      T fo[Q], u[3];
+      collide<S, T, SHIFT, COLL>(fh[0], dnh[0], gx, gy, gz, lpsi, P, fo, u);
+      const int mc = m0 + j - 2 * R;
+      const int xc = x, yc = D == 3 ? y : mc, zc = D == 3 ? mc : 0;
+      push_store<S, T>(pop_swap, geo, xc, yc, zc, fo);
+      if (WF) {
+        const int64_t g = lex(geo, xc, yc, zc);
+        const T dn = dnh[0];
+        n_out[g] = (double)(SHIFT ? P.rho0 + dn : dn);
+        psi_out[g] = (double)lpsi;
+        u_out[g] = (double)u[0];
+        u_out[g + geo.V] = (double)u[1];
+        if (D == 3) u_out[g + 2 * geo.V] = (double)u[2];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      dnh[k] = dnh[k + 1];
+#pragma unroll
+      for (int q = 0; q < Q; q++) fh[k][q] = fh[k + 1][q];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Two-pass kernels with the reference's structure.
+
+// K_ComputeDensityPsiMeta: n, psi from pop (MultiPhaseKernelsMeta.py:107-167).
+template <class S, class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_density_psi(const T* __restrict__ pop, Geom geo, Phys<T> P,
+              double* __restrict__ n, double* __restrict__ psi) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) s += pop[(size_t)q * geo.V + g];
+    const T ln = SHIFT ? P.rho0 + s : s;
+    n[g] = (double)ln;
+    if (psi) psi[g] = (double)psi_of(P.psi_kind, ln);
+  }
+}
+
+// Gradient of psi at (x, y, z) gathered from a global psi array.
+template <class S, class ES, class T>
+__device__ inline void grad_global(const double* __restrict__ psi, const Geom& geo,
+                                   int x, int y, int z, T& gx, T& gy, T& gz) {
+  gx = gy = gz = 0;
+#pragma unroll
+  for (int e = 0; e < ES::QE; e++) {
+    const int xn = ES::ex[e] ? wrap(x + ES::ex[e], geo.L0) : x;
+    const int yn = ES::ey[e] ? wrap(y + ES::ey[e], geo.L1) : y;
+    const int zn = ES::ez[e] ? wrap(z + ES::ez[e], geo.L2) : z;
+    const T v = (T)psi[lex(geo, xn, yn, zn)];
+    const T w = T(ES::w[e]);
+    if (ES::ex[e] != 0) gx += T(ES::ex[e]) * w * v;
+    if (ES::ey[e] != 0) gy += T(ES::ey[e]) * w * v;
+    if (ES::ez[e] != 0) gz += T(ES::ez[e]) * w * v;
+  }
+}
+
+__device__ inline void site_xyz(const Geom& geo, int64_t g, int& x, int& y, int& z) {
+  x = (int)(g % geo.L0);
+  const int64_t r = g / geo.L0;
+  y = (int)(r % geo.L1);
+  z = (int)(r / geo.L1);
+}
+
+// K_ForceCollideStreamSCMPMeta / _MRTMeta (MultiPhaseKernelsMeta.py:169-345,
+// 891-1075): force from the psi array, n from the n array, push stream.
+template <class S, class ES, class T, bool SHIFT, int COLL, bool WF>
+__global__ void __launch_bounds__(256)
+k_fcs(const T* __restrict__ pop, T* __restrict__ pop_swap,
+      const double* __restrict__ n_in, const double* __restrict__ psi_in,
+      Geom geo, Phys<T> P, double* __restrict__ u_out) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    site_xyz(geo, g, x, y, z);
+    T gx, gy, gz;
+    grad_global<S, ES, T>(psi_in, geo, x, y, z, gx, gy, gz);
+    T f[S::Q];
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) f[q] = pop[(size_t)q * geo.V + g];
+    const T ln = (T)n_in[g];
+    const T dn = SHIFT ? ln - P.rho0 : ln;
+    T fo[S::Q], u[3];
+    collide<S, T, SHIFT, COLL>(f, dn, gx, gy, gz, (T)psi_in[g], P, fo, u);
+    push_store<S, T>(pop_swap, geo, x, y, z, fo);
+    if (WF) {
+      u_out[g] = (double)u[0];
+      u_out[g + geo.V] = (double)u[1];
+      if (S::D == 3) u_out[g + 2 * geo.V] = (double)u[2];
+    }
+  }
+}
+
+// K_ComputeVelocityAfterForceSCMPMeta: u = sum xi f / n + F/(2n)
+// (MultiPhaseKernelsMeta.py:1077-1219), evaluated in double.
+template <class S, class ES, class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_velocity(const T* __restrict__ pop, const double* __restrict__ n_in,
+           const double* __restrict__ psi_in, Geom geo, double G,
+           double* __restrict__ u_out) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    site_xyz(geo, g, x, y, z);
+    double gx, gy, gz;
+    grad_global<S, ES, double>(psi_in, geo, x, y, z, gx, gy, gz);
+    double j[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) {
+      const double f = (double)pop[(size_t)q * geo.V + g];
+      j[0] += S::cx[q] * f;
+      j[1] += S::cy[q] * f;
+      j[2] += S::cz[q] * f;
+    }
+    const double ln = n_in[g];
+    const double mGpsi = -G * psi_in[g];
+    const double F[3] = {mGpsi * gx, mGpsi * gy, mGpsi * gz};
+    for (int d = 0; d < S::D; d++) u_out[g + d * geo.V] = j[d] / ln + 0.5 * F[d] / ln;
+  }
+}
+
+// K_InitPopulations: f = f_eq(n, u) (idpy/LBM/LBM.py:1700-1743).
+template <class S, class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_init_pop(T* __restrict__ pop, const double* __restrict__ n,
+           const double* __restrict__ u, Geom geo, double rho0) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const double ln = n[g];
+    const double ux = u[g], uy = u[g + geo.V], uz = S::D == 3 ? u[g + 2 * geo.V] : 0.0;
+    const double uu = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) {
+      const double xu = S::cx[q] * ux + S::cy[q] * uy + S::cz[q] * uz;
+      double e = 1.0;
+      e += xu * kCM2;
+      e += 0.5 * xu * xu * kCM4;
+      e -= 0.5 * uu * kCM2;
+      double f = e * ln * S::w[q];
+      if (SHIFT) f -= S::w[q] * rho0;
+      pop[(size_t)q * geo.V + g] = (T)f;
+    }
+  }
+}
+
+// Device population layout <-> float64 SoA (adds / removes the shift).
+template <class S, class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_pop_to_f64(const T* __restrict__ pop, double* __restrict__ dst, Geom geo, double rho0) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) {
+      double v = (double)pop[(size_t)q * geo.V + g];
+      if (SHIFT) v += S::w[q] * rho0;
+      dst[(size_t)q * geo.V + g] = v;
+    }
+  }
+}
+
+template <class S, class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_pop_from_f64(T* __restrict__ pop, const double* __restrict__ src, Geom geo, double rho0) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) {
+      double v = src[(size_t)q * geo.V + g];
+      if (SHIFT) v -= S::w[q] * rho0;
+      pop[(size_t)q * geo.V + g] = (T)v;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// FP64 reductions: wavefront (64-lane) shuffles, then LDS across waves.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Block of 256 threads = 4 waves. Returns the block total in thread 0.
+template <int OP>
+__device__ inline double block_reduce(double v) {
+  __shared__ double part[4];
+  v = OP == 0 ? wave_sum(v) : wave_max(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) part[wv] = v;
+  __syncthreads();
+  double r = part[0];
+  for (int k = 1; k < 4; k++) r = OP == 0 ? r + part[k] : fmax(r, part[k]);
+  __syncthreads();
+  return r;
+}
+
+// Per-block mass and momentum partials of a population buffer:
+// out[b*4 + 0] = sum n, out[b*4 + 1 + d] = sum j_d.
+template <class S, class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_mass_mom_partial(const T* __restrict__ pop, Geom geo, double rho0,
+                   double* __restrict__ out) {
+  double m = 0, jx = 0, jy = 0, jz = 0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0;
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) {
+      const double f = (double)pop[(size_t)q * geo.V + g];
+      s += f;
+      jx += S::cx[q] * f;
+      jy += S::cy[q] * f;
+      jz += S::cz[q] * f;
+    }
+    m += SHIFT ? s + rho0 : s;
+  }
+  m = block_reduce<0>(m);
+  jx = block_reduce<0>(jx);
+  jy = block_reduce<0>(jy);
+  jz = block_reduce<0>(jz);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x * 4 + 0] = m;
+    out[blockIdx.x * 4 + 1] = jx;
+    out[blockIdx.x * 4 + 2] = jy;
+    out[blockIdx.x * 4 + 3] = jz;
+  }
+}
+
+}  // namespace idmx

@@ -1,0 +1,109 @@
+"""idpy.LBM.MultiPhase surface: the current ShanChenMultiPhase class
+(idpy/LBM/MultiPhase.py:140-2175) on the HIP backend.
+
+MainLoop / MainLoopMRT run the fused Shan-Chen step (density, psi, E-stencil
+force, Guo-shifted velocity, BGK or MRT collision with Guo forcing, push
+streaming) on the device. Observable semantics follow the reference: after
+MainLoop, n and psi hold the values of the start of the last step and u is
+not written (call ComputeMomentsMeta first, MultiPhaseLoopChecks.py:125).
+"""
+import numpy as np
+
+from ..IdpyCode import HIP_T  # noqa: F401
+from ..IdpyCode import IdpyMemory
+from ._base import (AllTrue, IndexFromPos, OneTrue, PosFromIndex,  # noqa: F401
+                    _ShanChenCommon)
+from .stencils import FStencils, XIStencils  # noqa: F401
+
+
+def ComputeCenterOfMass(lbm, c_i=''):
+    """(mass, centre of mass) of n (idpy/LBM/MultiPhase.py:108-137)."""
+    return lbm.ComputeCenterOfMass()
+
+
+class ShanChenMultiPhase(_ShanChenCommon):
+    """Drop-in for idpy.LBM.MultiPhase.ShanChenMultiPhase.
+
+    kwargs as in SetupRoot (MultiPhase.py:2096-2108): lang, device,
+    custom_types, block_size, f_stencil, psi_code (C expression in ln_0),
+    SC_G, tau, optimizer_flag, e2_val, psi_sym, empty_sim, set_ordering,
+    use_ptrs, fp32_flag, dim_sizes, xi_stencil; plus precision ('fp64' |
+    'fp32s'), rho0 and step_mode ('fused' | 'two_pass'). fp32_flag selects
+    'fp32s'. The fluctuating-LBM kwargs are not supported.
+    """
+
+    _NEEDED = ['lang', 'cl_kind', 'device', 'custom_types', 'block_size', 'f_stencil',
+               'psi_code', 'SC_G', 'tau', 'optimizer_flag', 'e2_val', 'psi_sym',
+               'empty_sim', 'set_ordering', 'use_ptrs', 'fluctuations', 'prng_kind',
+               'init_seed', 'prng_init_from', 'prng_distribution', 'indep_gaussian',
+               'fp32_flag']
+
+    def __init__(self, *args, **kwargs):
+        self._setup(self._NEEDED, args, kwargs, 'ln_0')
+        if self.params_dict.get('fluctuations') is not None:
+            raise NotImplementedError('fluctuating LBM (Gross 2011) is not supported')
+        so = self.params_dict.get('set_ordering', None)
+        if so not in ('gpu', 'cpu', None):
+            raise Exception("Parameter 'set_ordering' must either be 'gpu' or 'cpu'")
+        self.params_dict['set_ordering'] = so if so is not None else self.tenet.GetKind()
+        self.params_dict.setdefault('use_ptrs', True)
+        self._init_vars()
+        self.sims_vars['ordering'] = \
+            self.sims_vars['ordering_lambdas'][self.params_dict['set_ordering']]
+        for name in ['n', 'u']:
+            self.sims_idpy_memory[name] = None
+        if not self.params_dict['empty_sim']:
+            aos = self.params_dict['set_ordering'] == 'cpu'
+            self._init_memory(pop_aos=aos, u_aos=aos)
+            # the step's own velocity goes to a private buffer: MainLoop does
+            # not write 'u' on the reference
+            self._u_step = IdpyMemory.Zeros(self.sims_vars['V'] * self.sims_vars['DIM'],
+                                            np.float64, self.tenet)
+        self.init_status = {'n': False, 'u': False, 'pop': False}
+
+    def MainLoop(self, time_steps, convergence_functions=[], profiling=False):
+        return self._run_loop(time_steps, convergence_functions, profiling, self.engine,
+                              self._u_step)
+
+    def MainLoopMRT(self, time_steps, relaxation_matrix=None, omega_syms_vals=None,
+                    convergence_functions=[], profiling=False):
+        """MRT time loop (MultiPhase.py:1311-1402). relaxation_matrix is the
+        Q x Q matrix S in the W-orthogonal Hermite moment order (a sympy
+        Matrix whose symbols omega_syms_vals = [(symbol, value), ...] fixes,
+        or a numeric array)."""
+        if relaxation_matrix is None:
+            raise Exception("Missing argument 'relaxation_matrix'")
+        if omega_syms_vals is None:
+            raise Exception("Missing argument 'omega_syms_vals'")
+        eng = self._mrt_engine(relaxation_matrix, omega_syms_vals)
+        return self._run_loop(time_steps, convergence_functions, profiling, eng,
+                              self._u_step)
+
+    def ComputeMomentsMeta(self):
+        """n, psi and the Guo-shifted u = sum xi f / n + F/(2n) on the device
+        (MultiPhase.py:1565-1662)."""
+        if not self.init_status['pop']:
+            raise Exception('Populations are not initialized')
+        m = self.sims_idpy_memory
+        self.engine.moments(m['pop'], m['n'], m['psi'], m['u'])
+        self.init_status['n'] = True
+        self.init_status['u'] = True
+
+    def InitRadialInterface(self, n_g, n_l, R, full_flag=True):
+        """K_InitRadialInterface, tanh(r - R) (MultiPhaseKernels.py:686-727)."""
+        self.sims_vars.update({'init_type': 'radial', 'n_g': n_g, 'n_l': n_l,
+                               'full_flag': full_flag, 'R': R})
+        self._set_n_u0(self._radial(n_g, n_l, R, 1.0, full_flag))
+        self.InitPopulations()
+
+    def InitSingleFlatInterface(self, n_g, n_l, direction=0, full_flag=True):
+        """K_InitSingleFlatInterface with F_NSingleFlatProfile
+        (MultiPhaseKernels.py:45-53, 650-684)."""
+        self.sims_vars.update({'init_type': 'flat', 'n_g': n_g, 'n_l': n_l,
+                               'full_flag': full_flag, 'direction': direction})
+        x = self._coords()[direction]
+        x0 = self.sims_vars['dim_center'][direction]
+        ff = int(bool(full_flag))
+        dn = ff * (n_l - n_g) + (1 - ff) * (n_g - n_l)
+        self._set_n_u0(n_g + dn * (np.tanh(x - x0) + 1.) * 0.5)
+        self.InitPopulations()

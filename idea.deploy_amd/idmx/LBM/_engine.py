@@ -1,0 +1,134 @@
+"""Thin handle around one idmx_sc_t (a Shan-Chen simulation in libidmx.so).
+
+Only parameter marshalling and device pointers live here; every arithmetic
+operation on lattice data runs in the HIP kernels.
+"""
+import ctypes as C
+
+import numpy as np
+
+from .. import _lib
+
+XI_IDS = {'D2Q9': 0, 'D3Q19': 1}
+E_IDS = {'E4': 4, 'E6': 6, 'E8': 8}
+PSI_KINDS = {'exp': 0, 'expinv': 1, 'linear': 2}
+PRECISIONS = {'fp64': 0, 'fp32s': 1}
+
+RUN_WRITE_FIELDS = 1
+RUN_TWO_PASS = 2
+
+
+class SCEngine:
+    def __init__(self, tenet, xi_name, e_name, dims, psi_kind, G, tau,
+                 precision='fp64', rho0=0.0, mrt=None):
+        self.tenet = tenet
+        self.xi_name, self.e_name = xi_name, e_name
+        self.dims = tuple(int(d) for d in dims)
+        self.D = len(self.dims)
+        self.Q = 9 if xi_name == 'D2Q9' else 19
+        self.V = int(np.prod(self.dims))
+        self.precision = precision
+        self.rho0 = float(rho0) if precision == 'fp32s' else 0.0
+        self.pop_dtype = np.float64 if precision == 'fp64' else np.float32
+        d = _lib.ScDesc()
+        d.xi = XI_IDS[xi_name]
+        d.estencil = E_IDS[e_name[2:]]
+        for k in range(3):
+            d.dims[k] = self.dims[k] if k < self.D else 1
+        d.psi_kind = PSI_KINDS[psi_kind]
+        d.SC_G = float(G)
+        d.tau = float(tau)
+        d.collision = 0 if mrt is None else 1
+        d.precision = PRECISIONS[precision]
+        d.rho0 = self.rho0
+        keep = []
+        if mrt is not None:
+            for name, arr in zip(('mrt_A', 'mrt_beta', 'mrt_gamma'), mrt):
+                a = np.ascontiguousarray(arr, dtype=np.float64)
+                keep.append(a)
+                setattr(d, name, a.ctypes.data)
+        h = C.c_void_p()
+        _lib.call('idmx_sc_create', tenet.handle, C.byref(d), C.byref(h))
+        self.handle = h
+        nb = C.c_size_t()
+        _lib.call('idmx_sc_pop_bytes', h, C.byref(nb))
+        self.pop_bytes = nb.value
+
+    def close(self):
+        if self.handle:
+            _lib.call('idmx_sc_destroy', self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- kernels ----------------------------------------------------------
+    def init_pop(self, pop, n, u):
+        _lib.call('idmx_sc_init_pop', self.handle, pop.ptr, n.ptr, u.ptr)
+
+    def density_psi(self, n, psi, pop):
+        _lib.call('idmx_sc_density_psi', self.handle, n.ptr, psi.ptr, pop.ptr)
+
+    def force_collide_stream(self, pop_swap, pop, n, psi):
+        _lib.call('idmx_sc_force_collide_stream', self.handle, pop_swap.ptr, pop.ptr,
+                  n.ptr, psi.ptr)
+
+    def run(self, pop, pop_swap, nsteps, n=None, psi=None, u=None,
+            write_fields=False, two_pass=False):
+        """Advances nsteps; returns True when the state ended in pop_swap's
+        allocation (the caller swaps its references, as M_SwapPop does)."""
+        a = C.c_void_p(pop.ptr.value)
+        b = C.c_void_p(pop_swap.ptr.value)
+        flags = (RUN_WRITE_FIELDS if write_fields else 0) | (RUN_TWO_PASS if two_pass else 0)
+        ptr = lambda x: x.ptr if x is not None else None
+        _lib.call('idmx_sc_run', self.handle, C.byref(a), C.byref(b), ptr(n), ptr(psi),
+                  ptr(u), int(nsteps), flags)
+        return a.value != pop.ptr.value
+
+    def moments(self, pop, n, psi, u):
+        _lib.call('idmx_sc_moments', self.handle, pop.ptr, n.ptr, psi.ptr, u.ptr)
+
+    def density_velocity(self, pop, n, u):
+        _lib.call('idmx_sc_density_velocity', self.handle, pop.ptr, n.ptr, u.ptr)
+
+    def mass_momentum(self, pop):
+        m = C.c_double()
+        mom = (C.c_double * 3)()
+        _lib.call('idmx_sc_mass_momentum', self.handle, pop.ptr, C.byref(m), mom)
+        return m.value, np.array(mom[:self.D])
+
+    def last_run_ms(self):
+        ms = C.c_double()
+        _lib.call('idmx_sc_last_run_ms', self.handle, C.byref(ms))
+        return ms.value
+
+    def set_launch(self, tx, H):
+        _lib.call('idmx_sc_set_launch', self.handle, int(tx), int(H))
+
+    def get_launch(self):
+        tx, ty, H = C.c_int(), C.c_int(), C.c_int()
+        _lib.call('idmx_sc_get_launch', self.handle, C.byref(tx), C.byref(ty), C.byref(H))
+        return tx.value, ty.value, H.value
+
+    # -- host views of a population buffer ---------------------------------
+    def pop_weights(self):
+        from .stencils import XIStencils
+        return np.array(XIStencils[self.xi_name]['Ws'], dtype=np.float64)
+
+    def pop_to_host_soa(self, pop):
+        """float64 SoA populations (Q, V) with the shift removed."""
+        from ..IdpyCode.IdpyMemory import IdpyArrayHIP
+        raw = IdpyArrayHIP.D2H(pop).reshape(self.Q, self.V).astype(np.float64)
+        if self.precision == 'fp32s':
+            raw += self.pop_weights()[:, None] * self.rho0
+        return raw
+
+    def pop_from_host_soa(self, pop, f):
+        f = np.asarray(f, dtype=np.float64).reshape(self.Q, self.V)
+        if self.precision == 'fp32s':
+            f = f - self.pop_weights()[:, None] * self.rho0
+        from ..IdpyCode.IdpyMemory import IdpyArrayHIP
+        IdpyArrayHIP.H2D(pop, f.astype(self.pop_dtype))

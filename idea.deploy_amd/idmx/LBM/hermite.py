@@ -1,0 +1,173 @@
+"""Hermite moment bases for the MRT collision, in exact rational arithmetic.
+
+The moment basis is the one idpy builds (idpy/IdpyStencils/IdpyStencils.py:
+905-1043): tensor Hermite polynomials of the lattice velocities, taken order by
+order in Taylor-tuple (combinations-with-replacement) order, dropping rows that
+vanish on the lattice, keeping the first Q linearly independent rows (the rref
+pivots); the W-orthogonal basis applies Gram-Schmidt with
+<a, b> = sum_i a_i b_i w_i to those rows.
+
+Derived here from the closed form of the one-dimensional factors,
+h_0 = 1, h_1 = x, h_{k+1}(x) = x h_k(x) - k c2 h_{k-1}(x)
+(c_s^k He_k(x/c_s) with c_s^2 = c2), so that a tensor component with index
+counts (k_x, k_y, k_z) is h_kx(xi_x) h_ky(xi_y) h_kz(xi_z).
+
+From a relaxation matrix S (in W-orthogonal moment order) the collision
+f* = M^-1 [S M f_eq + (1-S) M f] + M'^-1 (1 - S/2) M' F_guo
+(idpy/LBM/Collisions.py:96-160, idpy/LBM/Forcings.py:129-181; M' is the
+non-orthogonalised basis) is reduced to the three tables the kernel takes:
+A = M^-1 (1-S) M, beta_k = M^-1 S M v_k, gamma_k = M'^-1 (1-S/2) M' v_k with
+v_k = w * {1, xi_a, xi_a xi_b (a <= b)}.
+"""
+from fractions import Fraction as Fr
+from functools import lru_cache
+from itertools import combinations_with_replacement
+
+import numpy as np
+
+from .stencils import XIStencils
+
+C2 = Fr(1, 3)
+
+
+def _h1d(k, x):
+    a, b = Fr(1), Fr(x)
+    if k == 0:
+        return a
+    for j in range(1, k):
+        a, b = b, x * b - j * C2 * a
+    return b
+
+
+def _lattice(name):
+    st = XIStencils[name]
+    xis = st['XIs']
+    D = len(xis[0])
+    ws = [Fr(w).limit_denominator(1000) for w in st['Ws']]
+    return xis, ws, D
+
+
+def hermite_row(name, tup):
+    xis, _, D = _lattice(name)
+    counts = [tup.count(a) for a in range(D)]
+    row = []
+    for v in xis:
+        p = Fr(1)
+        for a in range(D):
+            p *= _h1d(counts[a], v[a])
+        row.append(p)
+    return row
+
+
+def _rank_filter(rows):
+    """Indices of the first linearly independent rows (greedy, exact)."""
+    basis = []   # reduced rows with their pivot column
+    keep = []
+    for i, r in enumerate(rows):
+        v = list(r)
+        for piv, b in basis:
+            if v[piv] != 0:
+                f = v[piv] / b[piv]
+                v = [x - f * y for x, y in zip(v, b)]
+        nz = [j for j, x in enumerate(v) if x != 0]
+        if nz:
+            basis.append((nz[0], v))
+            keep.append(i)
+    return keep
+
+
+@lru_cache(maxsize=None)
+def invertible_set(name, search_depth=6):
+    """(rows, tuples): the non-orthogonal invertible Hermite basis M'."""
+    xis, _, D = _lattice(name)
+    Q = len(xis)
+    rows, tuples = [[Fr(1)] * Q], [()]
+    order = 1
+    while len(rows) < Q + search_depth:
+        for tup in combinations_with_replacement(range(D), order):
+            r = hermite_row(name, tup)
+            if any(x != 0 for x in r):
+                rows.append(r)
+                tuples.append(tup)
+        order += 1
+    keep = _rank_filter(rows)
+    if len(keep) != Q:
+        raise RuntimeError('Hermite basis for %s has rank %d' % (name, len(keep)))
+    return tuple(tuple(rows[i]) for i in keep), tuple(tuples[i] for i in keep)
+
+
+@lru_cache(maxsize=None)
+def worth_set(name, search_depth=6):
+    """(rows, coeffs): W-orthogonal basis M and, for each row i, the
+    Gram-Schmidt coefficients c_ij with M_i = M'_i + sum_j c_ij M_j."""
+    rows, _ = invertible_set(name, search_depth)
+    _, ws, _ = _lattice(name)
+    prod = lambda a, b: sum(x * y * w for x, y, w in zip(a, b, ws))
+    out, coeffs = [list(rows[0])], [()]
+    for i in range(1, len(rows)):
+        v = list(rows[i])
+        cs = []
+        for u in out:
+            c = -prod(rows[i], u) / prod(u, u)
+            cs.append(c)
+            v = [a + c * b for a, b in zip(v, u)]
+        out.append(v)
+        coeffs.append(tuple(cs))
+    return tuple(tuple(r) for r in out), tuple(coeffs)
+
+
+def _inv(rows):
+    n = len(rows)
+    a = [list(r) + [Fr(int(i == j)) for j in range(n)] for i, r in enumerate(rows)]
+    for c in range(n):
+        p = next(r for r in range(c, n) if a[r][c] != 0)
+        a[c], a[p] = a[p], a[c]
+        pv = a[c][c]
+        a[c] = [x / pv for x in a[c]]
+        for r in range(n):
+            if r != c and a[r][c] != 0:
+                f = a[r][c]
+                a[r] = [x - f * y for x, y in zip(a[r], a[c])]
+    return [row[n:] for row in a]
+
+
+@lru_cache(maxsize=None)
+def _float_bases(name):
+    Mw, _ = worth_set(name)
+    Mp, _ = invertible_set(name)
+    f = lambda m: np.array([[float(x) for x in r] for r in m], dtype=np.float64)
+    return f(Mw), f(_inv(Mw)), f(Mp), f(_inv(Mp))
+
+
+def basis_vectors(name):
+    """v_k = w * {1, xi_a, xi_a xi_b (a <= b)} as an NB x Q float64 array, in
+    the order the kernel uses: 2D {1, x, y, xx, yy, xy}; 3D {1, x, y, z, xx,
+    yy, zz, xy, xz, yz}."""
+    st = XIStencils[name]
+    xi = np.array(st['XIs'], dtype=np.float64)
+    w = np.array(st['Ws'], dtype=np.float64)
+    D = xi.shape[1]
+    vs = [w.copy()]
+    vs += [w * xi[:, a] for a in range(D)]
+    vs += [w * xi[:, a] * xi[:, a] for a in range(D)]
+    vs += [w * xi[:, a] * xi[:, b] for a in range(D) for b in range(a + 1, D)]
+    return np.array(vs)
+
+
+def mrt_tables(name, S):
+    """(A, beta, gamma) float64 tables of the MRT kernel for relaxation
+    matrix S (Q x Q, W-orthogonal moment order)."""
+    Mw, Mwi, Mp, Mpi = _float_bases(name)
+    Q = Mw.shape[0]
+    S = np.asarray(S, dtype=np.float64)
+    if S.shape != (Q, Q):
+        raise ValueError('relaxation matrix must be %d x %d' % (Q, Q))
+    I = np.eye(Q)
+    A = Mwi @ (I - S) @ Mw
+    B = Mwi @ S @ Mw
+    G = Mpi @ (I - 0.5 * S) @ Mp
+    V = basis_vectors(name)
+    beta = (B @ V.T).T
+    gamma = (G @ V.T).T
+    return (np.ascontiguousarray(A), np.ascontiguousarray(beta),
+            np.ascontiguousarray(gamma))

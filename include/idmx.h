@@ -1,0 +1,202 @@
+/* idmx.h -- C ABI of libidmx.so, the MI355X (gfx950) Shan-Chen lattice
+ * Boltzmann library behind idpy's ShanChenMultiPhase surface.
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 on success, a negative code on error
+ *     (IDMX_E_*); idmx_last_error() returns the message of the calling
+ *     thread's last error. This replaces the reference's only error channel,
+ *     a RuntimeError raised when gcc fails (idpy/Utils/HostModule.py:226-230).
+ *   - Device buffers are opaque addresses (void*) owned by the caller
+ *     (allocated with idmx_alloc). Kernels borrow them. The reference passes
+ *     device handles the same way, as opaque pointers
+ *     (idpy/Utils/HostModule.py:273-285, idpy/Utils/CTypesTypes.py:63-66).
+ *   - Work is enqueued on the context's HIP stream. idmx_d2h blocks until the
+ *     copy is done; idmx_sync waits for all enqueued work.
+ *   - Lattice index g = x + y*L0 (+ z*L0*L1), x fastest
+ *     (idpy/IdpyCode/IdpyUnroll.py:325-386). Populations are SoA,
+ *     pop[g + q*V] (the reference's 'gpu' ordering, idpy/LBM/LBM.py:529-537).
+ *     Density n, pseudopotential psi and velocity u (u[g + d*V]) are float64.
+ *   - Populations are float64, or float32 "shifted" (stored value
+ *     f_q - w_q*rho0) when the simulation is created with IDMX_PREC_F32S.
+ */
+#ifndef IDMX_H
+#define IDMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IDMX_ABI_VERSION 1
+
+enum {
+  IDMX_OK = 0,
+  IDMX_E_ARG = -1,      /* invalid argument */
+  IDMX_E_HIP = -2,      /* HIP runtime error */
+  IDMX_E_UNSUPPORTED = -3,
+  IDMX_E_NOMEM = -4
+};
+
+/* Velocity sets (idpy/LBM/LBM.py:74-92). */
+enum { IDMX_D2Q9 = 0, IDMX_D3Q19 = 1 };
+/* Isotropic-gradient E-stencils, FStencils (idpy/LBM/LBM.py:94-185). */
+enum { IDMX_E4 = 4, IDMX_E6 = 6, IDMX_E8 = 8 };
+/* Pseudopotentials psi(n) (the psi_code/psi_sym kwargs,
+ * idpy/LBM/MultiPhase.py:2120): */
+enum {
+  IDMX_PSI_EXP = 0,    /* 1 - exp(-n) */
+  IDMX_PSI_EXPINV = 1, /* exp(-1/n)   */
+  IDMX_PSI_LINEAR = 2  /* n           */
+};
+enum { IDMX_BGK = 0, IDMX_MRT = 1 };
+enum { IDMX_PREC_F64 = 0, IDMX_PREC_F32S = 1 };
+
+/* Flags of idmx_sc_run. */
+enum {
+  IDMX_RUN_WRITE_FIELDS = 1, /* the last step also stores n, psi and the
+                                Guo-shifted u of that step */
+  IDMX_RUN_TWO_PASS = 2      /* use the unfused two-kernel step (density/psi
+                                kernel + force-collide-stream kernel), the
+                                reference's own structure */
+};
+
+typedef struct idmx_ctx_s* idmx_ctx_t;
+typedef struct idmx_sc_s* idmx_sc_t;
+
+/* Description of one Shan-Chen simulation (the constructor kwargs of
+ * ShanChenMultiPhase, idpy/LBM/MultiPhase.py:2096-2108). */
+typedef struct {
+  int xi;             /* IDMX_D2Q9 | IDMX_D3Q19                        */
+  int estencil;       /* IDMX_E4 | IDMX_E6 | IDMX_E8                   */
+  int dims[3];        /* L0, L1, L2 (L2 = 1 for D2Q9)                  */
+  int psi_kind;       /* IDMX_PSI_*                                    */
+  double SC_G;        /* coupling G                                    */
+  double tau;         /* BGK relaxation time; OMEGA = 1/tau            */
+  int collision;      /* IDMX_BGK | IDMX_MRT                           */
+  int precision;      /* IDMX_PREC_F64 | IDMX_PREC_F32S                */
+  double rho0;        /* population shift for IDMX_PREC_F32S           */
+  /* MRT only (collision == IDMX_MRT), row-major float64 tables built on
+   * the host from the relaxation matrix S (idpy/LBM/Collisions.py:96-160):
+   *   mrt_A     Q x Q     M^-1 (1 - S) M
+   *   mrt_beta  NB x Q    M^-1 S M v_k for the NB equilibrium basis vectors
+   *   mrt_gamma NB x Q    M'^-1 (1 - S/2) M' v_k for the same vectors
+   * with NB = 6 (D2Q9) or 10 (D3Q19), v_k = w∘{1, xi_a, xi_a xi_b (a<=b)}. */
+  const double* mrt_A;
+  const double* mrt_beta;
+  const double* mrt_gamma;
+} idmx_sc_desc;
+
+/* ---- library / context ---------------------------------------------- */
+int idmx_abi_version(void);
+/* Copies the calling thread's last error message into buf (NUL-terminated). */
+int idmx_last_error(char* buf, size_t len);
+int idmx_device_count(int* count);
+/* Creates a context on device `device` with its own HIP stream. Replaces the
+ * CTypes/CUDA Tenet (idpy/CTypes/CTypes.py:59-83, idpy/CUDA/CUDA.py:46). */
+int idmx_ctx_create(int device, idmx_ctx_t* out);
+int idmx_ctx_destroy(idmx_ctx_t ctx);
+int idmx_sync(idmx_ctx_t ctx);
+/* Device name into buf (the 'device_name' entry of IdpyLoopProfile,
+ * idpy/IdpyCode/IdpyCode.py:1913-2085). */
+int idmx_device_name(idmx_ctx_t ctx, char* buf, size_t len);
+
+/* ---- memory: IdpyMemory.Zeros/OnDevice, array.H2D/D2H
+ *      (idpy/IdpyCode/IdpyMemory.py:760-865) ----------------------------- */
+int idmx_alloc(idmx_ctx_t ctx, size_t bytes, void** dptr);
+int idmx_free(idmx_ctx_t ctx, void* dptr);
+int idmx_h2d(idmx_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int idmx_d2h(idmx_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int idmx_d2d(idmx_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int idmx_memset(idmx_ctx_t ctx, void* dst, int value, size_t bytes);
+
+/* ---- reductions, FP64 (IdpyMemory.Sum/Max, IdpyMemory.py:865-889) ----- */
+/* sum of n float64 values */
+int idmx_sum_f64(idmx_ctx_t ctx, const double* x, int64_t n, double* out);
+/* max of n float64 values */
+int idmx_max_f64(idmx_ctx_t ctx, const double* x, int64_t n, double* out);
+/* max_i |a_i - b_i| (the |delta rho|_max steady-state check) */
+int idmx_max_abs_diff_f64(idmx_ctx_t ctx, const double* a, const double* b,
+                          int64_t n, double* out);
+
+/* K_CheckUMeta followed by IdpyMemory.Sum/Max
+ * (idpy/LBM/MultiPhaseLoopChecks.py:96-163): *sum_delta = sum_g sum_d
+ * |u - old_u|, *max_u2 = max_g sum_d u^2; old_u is overwritten with u.
+ * u and old_u are SoA float64 with D components. */
+int idmx_check_u(idmx_ctx_t ctx, const double* u, double* old_u, int64_t V,
+                 int D, double* sum_delta, double* max_u2);
+
+/* ---- Shan-Chen simulation -------------------------------------------- */
+int idmx_sc_create(idmx_ctx_t ctx, const idmx_sc_desc* desc, idmx_sc_t* out);
+int idmx_sc_destroy(idmx_sc_t sim);
+/* Bytes of one population buffer (Q * V * sizeof(population)). */
+int idmx_sc_pop_bytes(idmx_sc_t sim, size_t* bytes);
+
+/* K_InitPopulations: pop = f_eq(n, u), 2nd-order Hermite equilibrium
+ * (idpy/LBM/LBM.py:1700-1743; K_InitPopulationsMeta
+ * idpy/LBM/LBMKernelsMeta.py:282-387). u is SoA float64. */
+int idmx_sc_init_pop(idmx_sc_t sim, void* pop, const double* n,
+                     const double* u);
+
+/* K_ComputeDensityPsiMeta(n, psi, pop): n = sum_q f_q, psi = psi(n)
+ * (idpy/LBM/MultiPhaseKernelsMeta.py:107-167). */
+int idmx_sc_density_psi(idmx_sc_t sim, double* n, double* psi,
+                        const void* pop);
+
+/* K_ForceCollideStreamSCMPMeta(pop_swap, pop, n, psi) and
+ * K_ForceCollideStreamSCMP_MRTMeta: Shan-Chen force from psi, Guo-shifted
+ * velocity, BGK or MRT collision with Guo forcing, push streaming into
+ * pop_swap (idpy/LBM/MultiPhaseKernelsMeta.py:169-345, 891-1075). */
+int idmx_sc_force_collide_stream(idmx_sc_t sim, void* pop_swap,
+                                 const void* pop, const double* n,
+                                 const double* psi);
+
+/* The MainLoop time loop (idpy/LBM/MultiPhase.py:1015-1097, legacy
+ * idpy/LBM/LBM.py:806-894): nsteps fused steps, each reading *pop and
+ * writing *pop_swap, then swapping the two addresses (M_SwapPop,
+ * idpy/LBM/LBMKernels.py:528-540). On return *pop holds the current state.
+ * With IDMX_RUN_WRITE_FIELDS, n, psi and u (Guo-shifted, SoA) of the last
+ * step are stored, which is what the reference's n/psi (both classes) and u
+ * (legacy class) arrays hold after MainLoop. n, psi, u may be NULL without
+ * that flag. */
+int idmx_sc_run(idmx_sc_t sim, void** pop, void** pop_swap, double* n,
+                double* psi, double* u, int nsteps, int flags);
+
+/* ComputeMomentsMeta: n, psi and u = sum xi f / n + F/(2n)
+ * (idpy/LBM/MultiPhase.py:1565-1662,
+ *  K_ComputeVelocityAfterForceSCMPMeta MultiPhaseKernelsMeta.py:1077-1219). */
+int idmx_sc_moments(idmx_sc_t sim, const void* pop, double* n, double* psi,
+                    double* u);
+
+/* K_ComputeMoments of the legacy class: n = sum_q f_q and the unshifted
+ * u = sum_q xi_q f_q / n (idpy/LBM/LBM.py:1667-1698). */
+int idmx_sc_density_velocity(idmx_sc_t sim, const void* pop, double* n,
+                             double* u);
+
+/* Total mass sum_g n(g) and momentum sum_g sum_q xi_q f_q(g) of a
+ * population buffer, FP64 accumulation; mom has D entries. */
+int idmx_sc_mass_momentum(idmx_sc_t sim, const void* pop, double* mass,
+                          double* mom);
+
+/* Converts between the device population layout and a float64 SoA host
+ * array (for FP32S simulations this adds/removes the w_q*rho0 shift). */
+int idmx_sc_pop_to_f64(idmx_sc_t sim, const void* pop, double* dst_dev);
+int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
+
+/* Launch shape of the fused step: tile width tx (2D: 64 or 256 sites per
+ * row; 3D: 32 for 32x8 tiles or 64 for 64x4 tiles) and H, the number of
+ * planes a block marches through. Chosen at creation; the autotuner
+ * (idmx.tune) overrides it. No reference counterpart: the reference fixes
+ * block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
+int idmx_sc_set_launch(idmx_sc_t sim, int tx, int H);
+int idmx_sc_get_launch(idmx_sc_t sim, int* tx, int* ty, int* H);
+
+/* Milliseconds of the last idmx_sc_run, measured with HIP events recorded
+ * on the simulation's stream around its kernels. */
+int idmx_sc_last_run_ms(idmx_sc_t sim, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IDMX_H */

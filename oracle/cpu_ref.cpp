@@ -1,0 +1,442 @@
+// cpu_ref.cpp -- CPU restatement of idpy's Shan-Chen multiphase LBM step.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the parity oracle and the CPU
+// baseline; it is never linked into, loaded by, or called from the product
+// library (libidmx.so). Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it.
+//
+// It restates, as a serial C++ loop over lattice sites (the CTypes backend runs
+// every kernel as `for(g_tid=0; g_tid<N_THREADS; g_tid++)`,
+// idpy/IdpyCode/IdpyCode.py:643-664), the arithmetic of:
+//   * the legacy class idpy.LBM.LBM.ShanChenMultiPhase (idpy/LBM/LBM.py:806-894)
+//     - K_ComputeMoments          idpy/LBM/LBM.py:1667-1698
+//     - K_ComputePsi              idpy/LBM/LBM.py:1613-1631
+//     - K_Collision_ShanChenGuoMultiPhase idpy/LBM/LBM.py:1529-1611
+//     - K_StreamPeriodic (pull)   idpy/LBM/LBM.py:1633-1665
+//     - M_SwapPop                 idpy/LBM/LBM.py:1921-1933
+//   * the current class idpy.LBM.MultiPhase.ShanChenMultiPhase.MainLoop
+//     (idpy/LBM/MultiPhase.py:1015-1097)
+//     - K_ComputeDensityPsiMeta   idpy/LBM/MultiPhaseKernelsMeta.py:107-167
+//     - K_ForceCollideStreamSCMPMeta (push) idpy/LBM/MultiPhaseKernelsMeta.py:169-345
+//   * MainLoopMRT (idpy/LBM/MultiPhase.py:1311-1402) with
+//     K_ForceCollideStreamSCMP_MRTMeta idpy/LBM/MultiPhaseKernelsMeta.py:891-1075
+//     f* = M^-1 [S M f_eq + (1-S) M f] + M'^-1 (1-S/2) M' F_guo
+//     (idpy/LBM/Collisions.py:96-160, idpy/LBM/Forcings.py:129-181); the three
+//     Q x Q products are formed once on the host by oracle/hermite_mrt.py.
+//   * ComputeMomentsMeta (idpy/LBM/MultiPhase.py:1565-1662,
+//     K_ComputeVelocityAfterForceSCMPMeta MultiPhaseKernelsMeta.py:1077-1219)
+//   * InitPopulations (K_InitPopulations idpy/LBM/LBM.py:1700-1743)
+//   * InitRadialInterface (legacy, idpy/LBM/LBM.py:1786-1826)
+//
+// Constants are full doubles ("exact" set): the legacy class reads W_list and
+// EW_list from memory as doubles (idpy/LBM/LBM.py:1596-1607), and the stencil
+// tables of idpy/LBM/LBM.py:74-185 are Python floats.
+//
+// Compile: g++ -std=c++17 -O3 -ffp-contract=off -fPIC -shared (oracle/Makefile).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- stencils
+// Velocity sets: idpy/LBM/LBM.py:74-92.
+struct SD2Q9 {
+  static constexpr int D = 2, Q = 9;
+  static constexpr int xi[9][3] = {{0, 0, 0},  {1, 0, 0},  {0, 1, 0},
+                                   {-1, 0, 0}, {0, -1, 0}, {1, 1, 0},
+                                   {-1, 1, 0}, {-1, -1, 0}, {1, -1, 0}};
+  static constexpr double w[9] = {4. / 9,  1. / 9,  1. / 9,  1. / 9, 1. / 9,
+                                  1. / 36, 1. / 36, 1. / 36, 1. / 36};
+};
+struct SD3Q19 {
+  static constexpr int D = 3, Q = 19;
+  static constexpr int xi[19][3] = {
+      {0, 0, 0},  {1, 0, 0},   {0, 1, 0},  {-1, 0, 0}, {0, -1, 0},
+      {0, 0, 1},  {0, 0, -1},  {1, 1, 0},  {-1, 1, 0}, {-1, -1, 0},
+      {1, -1, 0}, {1, 0, 1},   {-1, 0, 1}, {-1, 0, -1}, {1, 0, -1},
+      {0, 1, 1},  {0, -1, 1},  {0, -1, -1}, {0, 1, -1}};
+  static constexpr double w[19] = {
+      1. / 3,  1. / 18, 1. / 18, 1. / 18, 1. / 18, 1. / 18, 1. / 18,
+      1. / 36, 1. / 36, 1. / 36, 1. / 36, 1. / 36, 1. / 36, 1. / 36,
+      1. / 36, 1. / 36, 1. / 36, 1. / 36, 1. / 36};
+};
+constexpr double kC2 = 1. / 3;            // 'c2' of both velocity sets
+constexpr double kCM2 = 1. / kC2;         // CM2 = 1/c2   (LBM.py:542-569)
+constexpr double kCM4 = 1. / (kC2 * kC2); // CM4 = 1/c2^2
+
+// Pseudopotential forms accepted through psi_kind (psi_code is a C
+// expression in the reference, MultiPhase.py:2120, LBM.py:1244).
+inline double psi_of(int kind, double n) {
+  switch (kind) {
+  case 0: return 1. - exp(-n);   // psi = 1 - exp(-n)
+  case 1: return exp(-1. / n);   // psi = exp(-1/n)
+  default: return n;             // psi = n
+  }
+}
+
+// Lattice geometry with lexicographic index g = x + y*L0 + z*L0*L1
+// (idpy/IdpyCode/IdpyUnroll.py:325-386, idpy/LBM/LBM.py:410-425).
+struct Geo {
+  int D;
+  int L[3];
+  int64_t V;
+  int64_t idx(int x, int y, int z) const {
+    return (int64_t)x + (int64_t)L[0] * ((int64_t)y + (int64_t)L[1] * z);
+  }
+  void pos(int64_t g, int* p) const {
+    p[0] = (int)(g % L[0]);
+    p[1] = D > 1 ? (int)((g / L[0]) % L[1]) : 0;
+    p[2] = D > 2 ? (int)(g / ((int64_t)L[0] * L[1])) : 0;
+  }
+  // periodic neighbour, (p + e + L) % L per axis (LBM.py:1573-1575)
+  int64_t nb(const int* p, const int* e) const {
+    int q[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) q[d] = ((p[d] + e[d]) % L[d] + L[d]) % L[d];
+    return idx(q[0], q[1], q[2]);
+  }
+};
+
+// Population ordering: 'gpu' = SoA pop[g + q*V], 'cpu' = AoS pop[q + g*Q]
+// (idpy/LBM/LBM.py:529-537).
+struct Ord {
+  bool aos;
+  int64_t V;
+  int Q;
+  int64_t operator()(int64_t g, int q) const {
+    return aos ? (int64_t)q + g * Q : g + (int64_t)q * V;
+  }
+};
+
+// E-stencil passed at run time (FStencils, idpy/LBM/LBM.py:94-185).
+struct EStencil {
+  int QE;
+  const int* E;      // QE*D ints
+  const double* EW;  // QE weights
+};
+
+// Shan-Chen force at site g (IDShanChenForce.py:85-114, LBM.py:1562-1580):
+// F_d = -G psi(x) sum_e w_e e_d psi(x+e)
+template <int D>
+inline void sc_force(const Geo& geo, const EStencil& es, const double* psi,
+                     int64_t g, double G, double* F) {
+  int p[3];
+  geo.pos(g, p);
+  for (int d = 0; d < D; d++) F[d] = 0.;
+  for (int k = 0; k < es.QE; k++) {
+    int e[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) e[d] = es.E[k * D + d];
+    const double np = psi[geo.nb(p, e)];
+    for (int d = 0; d < D; d++) F[d] += e[d] * es.EW[k] * np;
+  }
+  const double lpsi = psi[g];
+  for (int d = 0; d < D; d++) F[d] *= -G * lpsi;
+}
+
+// Equilibrium + Guo term for population q (LBM.py:1582-1608;
+// Equilibria.py:107-138, Forcings.py:104-127 give the same polynomial).
+template <class S>
+inline void eq_guo(int q, double ln, const double* lu, const double* F,
+                   double u_dot_u, double F_dot_u, double* leq, double* lguo) {
+  double u_dot_xi = 0., F_dot_xi = 0.;
+  for (int d = 0; d < S::D; d++) {
+    u_dot_xi += lu[d] * S::xi[q][d];
+    F_dot_xi += F[d] * S::xi[q][d];
+  }
+  double e = 1.;
+  e += +u_dot_xi * kCM2 + 0.5 * u_dot_xi * u_dot_xi * kCM4;
+  e += -0.5 * u_dot_u * kCM2;
+  *leq = e * ln * S::w[q];
+  double gu = 0.;
+  gu += +F_dot_xi * kCM2 + F_dot_xi * u_dot_xi * kCM4;
+  gu += -F_dot_u * kCM2;
+  *lguo = gu * S::w[q];
+}
+
+// ------------------------------------------------ legacy class (SoA always)
+template <class S>
+void legacy_step(const Geo& geo, const EStencil& es, double G, double omega,
+                 int psi_kind, double*& pop, double*& pop_swap, double* n,
+                 double* u, double* psi) {
+  constexpr int D = S::D, Q = S::Q;
+  const int64_t V = geo.V;
+  // K_ComputeMoments (LBM.py:1667-1698)
+  for (int64_t g = 0; g < V; g++) {
+    double lu[3] = {0., 0., 0.};
+    double ln = 0.;
+    for (int q = 0; q < Q; q++) {
+      const double lpop = pop[g + q * V];
+      ln += lpop;
+      for (int d = 0; d < D; d++) lu[d] += lpop * S::xi[q][d];
+    }
+    n[g] = ln;
+    for (int d = 0; d < D; d++) u[g + d * V] = lu[d] / ln;
+  }
+  // K_ComputePsi (LBM.py:1613-1631)
+  for (int64_t g = 0; g < V; g++) psi[g] = psi_of(psi_kind, n[g]);
+  // K_Collision_ShanChenGuoMultiPhase (LBM.py:1529-1611)
+  for (int64_t g = 0; g < V; g++) {
+    double F[3];
+    sc_force<D>(geo, es, psi, g, G, F);
+    const double ln = n[g];
+    double lu[3];
+    for (int d = 0; d < D; d++) {
+      lu[d] = u[g + V * d] + 0.5 * F[d] / ln;
+      u[g + V * d] = lu[d];
+    }
+    double u_dot_u = 0., F_dot_u = 0.;
+    for (int d = 0; d < D; d++) {
+      u_dot_u += lu[d] * lu[d];
+      F_dot_u += F[d] * lu[d];
+    }
+    for (int q = 0; q < Q; q++) {
+      double leq, lguo;
+      eq_guo<S>(q, ln, lu, F, u_dot_u, F_dot_u, &leq, &lguo);
+      pop[g + q * V] = pop[g + q * V] * (1. - omega) + leq * omega +
+                       (1. - 0.5 * omega) * lguo;
+    }
+  }
+  // K_StreamPeriodic, pull (LBM.py:1633-1665)
+  for (int64_t g = 0; g < V; g++) {
+    int p[3];
+    geo.pos(g, p);
+    pop_swap[g] = pop[g];
+    for (int q = 1; q < Q; q++) {
+      int e[3] = {-S::xi[q][0], -S::xi[q][1], -S::xi[q][2]};
+      pop_swap[g + q * V] = pop[geo.nb(p, e) + q * V];
+    }
+  }
+  // M_SwapPop (LBM.py:1921-1933)
+  double* t = pop;
+  pop = pop_swap;
+  pop_swap = t;
+}
+
+// ------------------------------------------- Meta class, BGK or MRT (push)
+struct MRTMats {
+  const double* A;   // M^-1 (1-S) M     (Q x Q, row-major)
+  const double* B;   // M^-1 S M
+  const double* Gg;  // M'^-1 (1-S/2) M'
+};
+
+template <class S>
+void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
+               double omega, int psi_kind, const MRTMats* mrt, double*& pop,
+               double*& pop_swap, double* n, double* psi) {
+  constexpr int D = S::D, Q = S::Q;
+  const int64_t V = geo.V;
+  // K_ComputeDensityPsiMeta (MultiPhaseKernelsMeta.py:107-167)
+  for (int64_t g = 0; g < V; g++) {
+    double ln = 0.;
+    for (int q = 0; q < Q; q++) ln += pop[ord(g, q)];
+    psi[g] = psi_of(psi_kind, ln);
+    n[g] = ln;
+  }
+  // K_ForceCollideStreamSCMPMeta / _MRTMeta (MultiPhaseKernelsMeta.py:169-345,
+  // 891-1075)
+  for (int64_t g = 0; g < V; g++) {
+    double F[3];
+    sc_force<D>(geo, es, psi, g, G, F);
+    const double ln = n[g];
+    double lpop[Q];
+    double lu[3] = {0., 0., 0.};
+    for (int q = 0; q < Q; q++) {
+      lpop[q] = pop[ord(g, q)];
+      for (int d = 0; d < D; d++) lu[d] += lpop[q] * S::xi[q][d];
+    }
+    // VelocityCode (IdpyStencils.py:444-561) then Guo shift u += F/(2n)
+    // (Forcings.py:309-329)
+    for (int d = 0; d < D; d++) lu[d] = lu[d] / ln + 0.5 * F[d] / ln;
+    double u_dot_u = 0., F_dot_u = 0.;
+    for (int d = 0; d < D; d++) {
+      u_dot_u += lu[d] * lu[d];
+      F_dot_u += F[d] * lu[d];
+    }
+    double feq[Q], fg[Q], fout[Q];
+    for (int q = 0; q < Q; q++) eq_guo<S>(q, ln, lu, F, u_dot_u, F_dot_u, &feq[q], &fg[q]);
+    if (!mrt) {
+      // SRT: (1-w) f + w f_eq + (1-w/2) F_guo (Collisions.py:163-185)
+      for (int q = 0; q < Q; q++)
+        fout[q] = lpop[q] * (1. - omega) + feq[q] * omega + (1. - 0.5 * omega) * fg[q];
+    } else {
+      for (int q = 0; q < Q; q++) {
+        double a = 0.;
+        for (int k = 0; k < Q; k++)
+          a += mrt->A[q * Q + k] * lpop[k] + mrt->B[q * Q + k] * feq[k] +
+               mrt->Gg[q * Q + k] * fg[k];
+        fout[q] = a;
+      }
+    }
+    // push stream: pop_swap[x + xi_q, q] = f*_q (Collisions.py:487-530)
+    int p[3];
+    geo.pos(g, p);
+    for (int q = 0; q < Q; q++) {
+      int e[3] = {S::xi[q][0], S::xi[q][1], S::xi[q][2]};
+      pop_swap[ord(geo.nb(p, e), q)] = fout[q];
+    }
+  }
+  double* t = pop;
+  pop = pop_swap;
+  pop_swap = t;
+}
+
+// ComputeMomentsMeta: n, psi, then u = sum xi f / n + F/(2n)
+// (MultiPhase.py:1565-1662, MultiPhaseKernelsMeta.py:1077-1219)
+template <class S>
+void meta_moments(const Geo& geo, const EStencil& es, const Ord& ordp,
+                  bool u_aos, double G, int psi_kind, const double* pop,
+                  double* n, double* u, double* psi) {
+  constexpr int D = S::D, Q = S::Q;
+  const int64_t V = geo.V;
+  for (int64_t g = 0; g < V; g++) {
+    double ln = 0.;
+    for (int q = 0; q < Q; q++) ln += pop[ordp(g, q)];
+    psi[g] = psi_of(psi_kind, ln);
+    n[g] = ln;
+  }
+  for (int64_t g = 0; g < V; g++) {
+    double F[3];
+    sc_force<D>(geo, es, psi, g, G, F);
+    double lu[3] = {0., 0., 0.};
+    for (int q = 0; q < Q; q++) {
+      const double f = pop[ordp(g, q)];
+      for (int d = 0; d < D; d++) lu[d] += f * S::xi[q][d];
+    }
+    for (int d = 0; d < D; d++) {
+      const double v = lu[d] / n[g] + 0.5 * F[d] / n[g];
+      u[u_aos ? d + g * D : g + d * V] = v;
+    }
+  }
+}
+
+// K_InitPopulations: f = f_eq(n, u) order 2 (LBM.py:1700-1743)
+template <class S>
+void init_pop(const Geo& geo, const Ord& ord, const double* n, const double* u,
+              double* pop) {
+  constexpr int D = S::D, Q = S::Q;
+  const int64_t V = geo.V;
+  for (int64_t g = 0; g < V; g++) {
+    const double ln = n[g];
+    double lu[3] = {0., 0., 0.}, u_dot_u = 0.;
+    for (int d = 0; d < D; d++) {
+      lu[d] = u[g + d * V];
+      u_dot_u += lu[d] * lu[d];
+    }
+    for (int q = 0; q < Q; q++) {
+      double u_dot_xi = 0.;
+      for (int d = 0; d < D; d++) u_dot_xi += lu[d] * S::xi[q][d];
+      double e = 1.;
+      e += u_dot_xi * kCM2;
+      e += 0.5 * u_dot_xi * u_dot_xi * kCM4;
+      e -= 0.5 * u_dot_u * kCM2;
+      pop[ord(g, q)] = e * ln * S::w[q];
+    }
+  }
+}
+
+bool make_geo(int D, const int* L, Geo* geo) {
+  if (D < 2 || D > 3) return false;
+  geo->D = D;
+  geo->L[0] = L[0];
+  geo->L[1] = L[1];
+  geo->L[2] = D > 2 ? L[2] : 1;
+  geo->V = (int64_t)geo->L[0] * geo->L[1] * geo->L[2];
+  for (int d = 0; d < D; d++)
+    if (L[d] <= 0) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Velocity set ids: 0 = D2Q9, 1 = D3Q19.
+// Runs nsteps of the legacy class MainLoop (SoA). After the call pop holds
+// the state (the function copies it back when the swap count is odd); n, u,
+// psi hold the values computed during the last step, as on the reference.
+int cpuref_legacy_run(int xi_id, const int* L, int QE, const int* E,
+                      const double* EW, double G, double omega, int psi_kind,
+                      double* pop, double* pop_swap, double* n, double* u,
+                      double* psi, int nsteps) {
+  Geo geo;
+  if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
+  EStencil es{QE, E, EW};
+  double *a = pop, *b = pop_swap;
+  for (int s = 0; s < nsteps; s++) {
+    if (xi_id == 0) legacy_step<SD2Q9>(geo, es, G, omega, psi_kind, a, b, n, u, psi);
+    else legacy_step<SD3Q19>(geo, es, G, omega, psi_kind, a, b, n, u, psi);
+  }
+  const int Q = xi_id == 0 ? 9 : 19;
+  if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
+  return 0;
+}
+
+// Runs nsteps of the Meta class MainLoop (mrt == 0) or MainLoopMRT (A, B, Gg
+// given). aos != 0 selects the CTypes 'cpu' ordering pop[q + g*Q].
+int cpuref_meta_run(int xi_id, const int* L, int QE, const int* E,
+                    const double* EW, double G, double omega, int psi_kind,
+                    int aos, const double* A, const double* B, const double* Gg,
+                    double* pop, double* pop_swap, double* n, double* psi,
+                    int nsteps) {
+  Geo geo;
+  if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
+  const int Q = xi_id == 0 ? 9 : 19;
+  EStencil es{QE, E, EW};
+  Ord ord{aos != 0, geo.V, Q};
+  MRTMats m{A, B, Gg};
+  const MRTMats* mp = A ? &m : nullptr;
+  double *a = pop, *b = pop_swap;
+  for (int s = 0; s < nsteps; s++) {
+    if (xi_id == 0) meta_step<SD2Q9>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi);
+    else meta_step<SD3Q19>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi);
+  }
+  if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
+  return 0;
+}
+
+int cpuref_meta_moments(int xi_id, const int* L, int QE, const int* E,
+                        const double* EW, double G, int psi_kind, int aos,
+                        const double* pop, double* n, double* u, double* psi) {
+  Geo geo;
+  if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
+  const int Q = xi_id == 0 ? 9 : 19;
+  EStencil es{QE, E, EW};
+  Ord ord{aos != 0, geo.V, Q};
+  if (xi_id == 0) meta_moments<SD2Q9>(geo, es, ord, aos != 0, G, psi_kind, pop, n, u, psi);
+  else meta_moments<SD3Q19>(geo, es, ord, aos != 0, G, psi_kind, pop, n, u, psi);
+  return 0;
+}
+
+int cpuref_init_pop(int xi_id, const int* L, int aos, const double* n,
+                    const double* u, double* pop) {
+  Geo geo;
+  if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
+  const int Q = xi_id == 0 ? 9 : 19;
+  Ord ord{aos != 0, geo.V, Q};
+  if (xi_id == 0) init_pop<SD2Q9>(geo, ord, n, u, pop);
+  else init_pop<SD3Q19>(geo, ord, n, u, pop);
+  return 0;
+}
+
+// Legacy K_InitRadialInterface: n = (ng+nl)/2 - (nl-ng)/2 tanh((r-R)/W),
+// centre L//2 (LBM.py:1786-1826, 422-423; MultiPhaseKernels.py:99-112).
+int cpuref_init_radial(int D, const int* L, double n_g, double n_l, double R,
+                       double W, double* n, double* u) {
+  Geo geo;
+  if (!make_geo(D, L, &geo)) return -1;
+  int c[3] = {geo.L[0] / 2, geo.L[1] / 2, geo.L[2] / 2};
+  const double delta_n = n_l - n_g;
+  for (int64_t g = 0; g < geo.V; g++) {
+    int p[3];
+    geo.pos(g, p);
+    double dist = 0;
+    for (int d = 0; d < D; d++) dist += (c[d] - p[d]) * (c[d] - p[d]);
+    const double r = sqrt(dist);
+    n[g] = 0.5 * (n_g + n_l) - 0.5 * delta_n * tanh((r - R) / W);
+    for (int d = 0; d < D; d++) u[g + d * geo.V] = 0.;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+"""ctypes wrapper of the serial C++ oracle libcpuref.so (TEST INFRASTRUCTURE).
+
+Builds nothing: oracle/Makefile makes libcpuref.so (build() of
+__graft_entry__ runs it). Arrays are numpy float64, C-contiguous.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .stencils import E, XI
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags='C_CONTIGUOUS')
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags='C_CONTIGUOUS')
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, 'libcpuref.so')
+        if not os.path.exists(path):
+            raise RuntimeError('oracle/libcpuref.so is not built (run make -C oracle)')
+        L = C.CDLL(path)
+        L.cpuref_legacy_run.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
+                                        C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp,
+                                        C.c_int]
+        L.cpuref_meta_run.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]
+        L.cpuref_meta_moments.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
+                                          C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.cpuref_init_pop.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp]
+        L.cpuref_init_radial.argtypes = [C.c_int, _ip, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _xi_id(xi):
+    return {'D2Q9': 0, 'D3Q19': 1}[xi]
+
+
+def _estencil(estencil):
+    es, ws = E[estencil]
+    Ei = np.ascontiguousarray(np.array(es, dtype=np.int32).ravel())
+    Ew = np.ascontiguousarray(np.array([float(w) for w in ws], dtype=np.float64))
+    return len(es), Ei, Ew
+
+
+def _L(dims):
+    return np.ascontiguousarray(np.array(list(dims) + [1] * (3 - len(dims)), dtype=np.int32))
+
+
+PSI_KIND = {'exp': 0, 'expinv': 1, 'linear': 2}
+
+
+class State:
+    """Fields of one oracle simulation (pop in the given ordering)."""
+
+    def __init__(self, xi, dims, aos=False):
+        self.xi, self.dims, self.aos = xi, tuple(dims), aos
+        self.Q = len(XI[xi])
+        self.D = len(dims)
+        self.V = int(np.prod(dims))
+        self.pop = np.zeros(self.V * self.Q)
+        self.pop_swap = np.zeros(self.V * self.Q)
+        self.n = np.zeros(self.V)
+        self.psi = np.zeros(self.V)
+        self.u = np.zeros(self.V * self.D)
+
+
+def init_radial(dims, n_g, n_l, R, W):
+    """Legacy K_InitRadialInterface (idpy/LBM/LBM.py:1786-1826)."""
+    V = int(np.prod(dims))
+    n = np.zeros(V)
+    u = np.zeros(V * len(dims))
+    rc = lib().cpuref_init_radial(len(dims), _L(dims), n_g, n_l, R, W, n, u)
+    assert rc == 0
+    return n, u
+
+
+def init_pop(st, n, u):
+    """K_InitPopulations (idpy/LBM/LBM.py:1700-1743); u is SoA."""
+    rc = lib().cpuref_init_pop(_xi_id(st.xi), _L(st.dims), int(st.aos),
+                               np.ascontiguousarray(n, dtype=np.float64),
+                               np.ascontiguousarray(u, dtype=np.float64), st.pop)
+    assert rc == 0
+
+
+def legacy_run(st, estencil, G, tau, psi='exp', nsteps=1):
+    """Legacy class MainLoop, nsteps (idpy/LBM/LBM.py:806-894). SoA only."""
+    assert not st.aos
+    QE, Ei, Ew = _estencil(estencil)
+    rc = lib().cpuref_legacy_run(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, 1.0 / tau,
+                                 PSI_KIND[psi], st.pop, st.pop_swap, st.n, st.u,
+                                 st.psi, nsteps)
+    assert rc == 0
+
+
+def meta_run(st, estencil, G, tau, psi='exp', nsteps=1, mrt=None):
+    """Meta class MainLoop / MainLoopMRT, nsteps
+    (idpy/LBM/MultiPhase.py:1015-1097, 1311-1402). mrt = (A, B, Gg)."""
+    QE, Ei, Ew = _estencil(estencil)
+    ptrs = [None, None, None]
+    keep = []
+    if mrt is not None:
+        for i, m in enumerate(mrt):
+            a = np.ascontiguousarray(m, dtype=np.float64)
+            keep.append(a)
+            ptrs[i] = a.ctypes.data
+    rc = lib().cpuref_meta_run(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, 1.0 / tau,
+                               PSI_KIND[psi], int(st.aos), ptrs[0], ptrs[1], ptrs[2],
+                               st.pop, st.pop_swap, st.n, st.psi, nsteps)
+    assert rc == 0
+
+
+def meta_moments(st, estencil, G, psi='exp'):
+    """ComputeMomentsMeta (idpy/LBM/MultiPhase.py:1565-1662): fills n, psi, u
+    with u = sum xi f / n + F / (2n); u in the same ordering as pop."""
+    QE, Ei, Ew = _estencil(estencil)
+    rc = lib().cpuref_meta_moments(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G,
+                                   PSI_KIND[psi], int(st.aos), st.pop, st.n, st.u, st.psi)
+    assert rc == 0

@@ -1,0 +1,84 @@
+"""numpy brute force of one Shan-Chen step (TEST INFRASTRUCTURE ONLY).
+
+An independent, vectorised restatement (np.roll over lattice and E vectors)
+used only to pin the C++ oracle on small grids, following the pattern of the
+reference's periodic-convolution check (idpy/test_convolution.py:92-160).
+Fields are numpy arrays of shape flip(dims) (z, y, x), populations have a
+leading Q axis.
+"""
+import numpy as np
+
+from .stencils import E, XI, XI_W
+
+
+def _shift(a, e):
+    """a(x + e) on the periodic lattice; e is (e_x, e_y[, e_z])."""
+    D = len(e)
+    return np.roll(a, shift=tuple(-int(v) for v in e), axis=tuple(range(a.ndim - 1, a.ndim - 1 - D, -1)))
+
+
+def psi_fn(n, kind='exp'):
+    return {'exp': lambda n: 1.0 - np.exp(-n),
+            'expinv': lambda n: np.exp(-1.0 / n),
+            'linear': lambda n: n}[kind](n)
+
+
+def sc_force(psi, estencil, G):
+    """F_d = -G psi(x) sum_e w_e e_d psi(x+e) (IDShanChenForce.py:85-114)."""
+    es, ws = E[estencil]
+    D = len(es[0])
+    grad = [np.zeros_like(psi) for _ in range(D)]
+    for e, w in zip(es, ws):
+        sp_ = _shift(psi, e)
+        for d in range(D):
+            if e[d]:
+                grad[d] += e[d] * float(w) * sp_
+    return [-G * psi * g for g in grad]
+
+
+def feq_guo(xi, n, u, F):
+    """Second-order Hermite equilibrium and Guo term without (1 - w/2)
+    (Equilibria.py:107-138, Forcings.py:104-127)."""
+    c2 = 1.0 / 3.0
+    uu = sum(ud * ud for ud in u)
+    Fu = sum(Fd * ud for Fd, ud in zip(F, u))
+    feq, fg = [], []
+    for v, w in zip(XI[xi], XI_W[xi]):
+        w = float(w)
+        xu = sum(v[d] * u[d] for d in range(len(u)))
+        xF = sum(v[d] * F[d] for d in range(len(u)))
+        feq.append(w * n * (1 + xu / c2 + xu * xu / (2 * c2 * c2) - uu / (2 * c2)))
+        fg.append(w * (xF / c2 + xF * xu / (c2 * c2) - Fu / c2))
+    return np.array(feq), np.array(fg)
+
+
+def step(f, xi, estencil, G, tau, psi_kind='exp', mrt=None):
+    """One step: density/psi, force, Guo-shifted velocity, BGK (or the
+    dense MRT form A f + B f_eq + Gg F_guo), push streaming. Returns
+    (f_new, n, psi, u_shifted)."""
+    omega = 1.0 / tau
+    xis = XI[xi]
+    D = len(xis[0])
+    n = f.sum(axis=0)
+    psi = psi_fn(n, psi_kind)
+    F = sc_force(psi, estencil, G)
+    u = [sum(xis[q][d] * f[q] for q in range(len(xis))) / n + 0.5 * F[d] / n for d in range(D)]
+    feq, fg = feq_guo(xi, n, u, F)
+    if mrt is None:
+        fs = (1 - omega) * f + omega * feq + (1 - 0.5 * omega) * fg
+    else:
+        A, B, Gg = mrt
+        fs = (np.tensordot(A, f, axes=1) + np.tensordot(B, feq, axes=1)
+              + np.tensordot(Gg, fg, axes=1))
+    fn = np.empty_like(f)
+    for q, v in enumerate(xis):
+        fn[q] = _shift(fs[q], tuple(-c for c in v))
+    return fn, n, psi, u
+
+
+def init_pop(xi, n, u=None):
+    D = len(XI[xi][0])
+    if u is None:
+        u = [np.zeros_like(n)] * D
+    feq, _ = feq_guo(xi, n, u, [np.zeros_like(n)] * D)
+    return feq

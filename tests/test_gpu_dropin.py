@@ -1,0 +1,87 @@
+"""The drop-in classes, constructed as a notebook does, on the GPU: the
+legacy idpy.LBM.LBM.ShanChenMultiPhase surface and the Meta
+idpy.LBM.MultiPhase.ShanChenMultiPhase surface, compared with the oracle
+(workload: D2Q9 128^2 droplet with seeded noise, E4, G = -5, tau = 1)."""
+import numpy as np
+import pytest
+import sympy as sp
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _legacy(nsteps_list, **kw):
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.LBM import FStencils, ShanChenMultiPhase, XIStencils
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=(128, 128), xi_stencil=XIStencils['D2Q9'],
+                             f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln)',
+                             psi_sym=1 - sp.exp(-n), SC_G=-5.0 / 3.0, tau=1.0, **kw)
+    return sim
+
+
+def test_legacy_class_c1_matches_oracle():
+    from idmx import configs
+    from oracle import cpuref
+    n0 = configs.droplet2d_noise()
+    sim = _legacy(None)
+    sim.SetDensityField(n0)
+    sim.MainLoop([0, 100, 300])
+    st = cpuref.State('D2Q9', (128, 128))
+    cpuref.init_pop(st, n0, np.zeros(2 * 128 * 128))
+    cpuref.legacy_run(st, 'D2E4', -5.0 / 3.0, 1.0, nsteps=300)
+    assert rel_err(sim.GetDensityField().ravel(), st.n) <= 1e-12
+    assert rel_err(sim.sims_idpy_memory['psi'].D2H(), st.psi) <= 1e-12
+    assert rel_err(sim.sims_idpy_memory['u'].D2H(), st.u) <= 1e-11
+    assert rel_err(sim.sims_idpy_memory['pop'].D2H(), st.pop) <= 1e-12
+    sim.End()
+
+
+@pytest.mark.parametrize('ordering', ['gpu', 'cpu'])
+def test_meta_class_orderings(ordering):
+    from idmx import configs
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    from oracle import cpuref
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=(128, 128), xi_stencil=XIStencils['D2Q9'],
+                             f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=-5.0 / 3.0, tau=1.0,
+                             set_ordering=ordering)
+    n0 = configs.droplet2d_noise()
+    sim.SetDensityField(n0)
+    sim.MainLoop([0, 200])
+    sim.ComputeMomentsMeta()
+    st = cpuref.State('D2Q9', (128, 128), aos=(ordering == 'cpu'))
+    ref0 = cpuref.State('D2Q9', (128, 128))
+    cpuref.init_pop(ref0, n0, np.zeros(2 * 128 * 128))
+    st.pop[:] = ref0.pop.reshape(9, -1).T.ravel() if ordering == 'cpu' else ref0.pop
+    cpuref.meta_run(st, 'D2E4', -5.0 / 3.0, 1.0, nsteps=200)
+    cpuref.meta_moments(st, 'D2E4', -5.0 / 3.0)
+    assert rel_err(sim.GetDensityField().ravel(), st.n) <= 1e-12
+    assert rel_err(sim.GetVelocityField().ravel(), st.u) <= 1e-11
+    pops = sim.GetPopulations()
+    assert pops.shape == ((128, 128, 9) if ordering == 'cpu' else (9, 128, 128))
+    assert rel_err(pops.ravel(), st.pop) <= 1e-12
+    sim.End()
+
+
+def test_meta_mainloop_does_not_write_u_and_checks_run():
+    from idmx import configs
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    from idmx.LBM.MultiPhaseLoopChecks import CheckUConvergenceSCMP
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=(64, 64), xi_stencil=XIStencils['D2Q9'],
+                             f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=-5.0 / 3.0, tau=1.0)
+    sim.SetDensityField(configs.droplet((64, 64), 12.0))
+    sim.MainLoop([0, 50])
+    assert np.all(sim.sims_idpy_memory['u'].D2H() == 0.0)
+    sim.MainLoop([0, 100, 200, 300], convergence_functions=[CheckUConvergenceSCMP])
+    assert len(sim.sims_vars['max_u']) == 3
+    assert 0 < sim.sims_vars['max_u'][-1] < 0.1
+    p_in, p_out, dp = sim.DeltaPLaplace()
+    assert dp > 0
+    sim.End()

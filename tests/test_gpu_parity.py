@@ -1,0 +1,144 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (FP64, 1e-12).
+
+The HIP kernels are reached through the C ABI (SCEngine wraps idmx_sc_*);
+the oracle is oracle/libcpuref.so. Sizes are small and deliberately not
+multiples of the tile shapes, so partial tiles and periodic wrap of the halo
+are exercised.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+from helpers import droplet_field, rel_err
+
+G = -5.0 / 3.0
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+CASES = [
+    ('D2Q9', 'D2E4', (37, 29)),
+    ('D2Q9', 'D2E6', (40, 23)),
+    ('D2Q9', 'D2E8', (300, 17)),
+    ('D2Q9', 'D2E8', (13, 70)),
+    ('D3Q19', 'D3E4', (13, 11, 9)),
+    ('D3Q19', 'D3E6', (35, 10, 7)),
+    ('D3Q19', 'D3E8', (9, 12, 11)),
+    ('D3Q19', 'D3E4', (70, 9, 5)),
+]
+
+
+def _device_fields(tenet, eng, n0):
+    from idmx.IdpyCode import IdpyMemory
+    from idmx.LBM._base import PopArray
+    V, D = eng.V, eng.D
+    n = IdpyMemory.OnDevice(n0, tenet)
+    u = IdpyMemory.Zeros(V * D, np.float64, tenet)
+    psi = IdpyMemory.Zeros(V, np.float64, tenet)
+    pop = PopArray(eng, tenet)
+    swap = PopArray(eng, tenet)
+    eng.init_pop(pop, n, u)
+    return pop, swap, n, psi, u
+
+
+def _oracle(xi, es, dims, n0, nsteps, G=G, tau=1.0, mrt=None):
+    from oracle import cpuref
+    V, D = int(np.prod(dims)), len(dims)
+    st = cpuref.State(xi, dims)
+    cpuref.init_pop(st, n0, np.zeros(V * D))
+    cpuref.meta_run(st, es, G, tau, nsteps=nsteps, mrt=mrt)
+    n1, u1, psi1 = st.n.copy(), None, st.psi.copy()
+    mom = cpuref.State(xi, dims)
+    mom.pop[:] = st.pop
+    cpuref.meta_moments(mom, es, G)
+    return st.pop.copy(), n1, psi1, mom
+
+
+@pytest.mark.parametrize('xi,es,dims', CASES)
+@pytest.mark.parametrize('mode', ['fused', 'two_pass'])
+def test_bgk_fp64_matches_oracle(tenet, xi, es, dims, mode):
+    from idmx.LBM._engine import SCEngine
+    print('case', xi, es, dims, mode, flush=True)
+    n0 = droplet_field(dims, seed=zlib.crc32(repr((xi, es, dims)).encode()) % 1000)
+    nsteps = 40
+    eng = SCEngine(tenet, xi, es, dims, 'exp', G, 1.0)
+    pop, swap, n, psi, u = _device_fields(tenet, eng, n0)
+    if eng.run(pop, swap, nsteps, n=n, psi=psi, u=u, write_fields=True,
+               two_pass=(mode == 'two_pass')):
+        pop, swap = swap, pop
+    f_gpu = eng.pop_to_host_soa(pop).ravel()
+    f_ref, n_ref, psi_ref, mom = _oracle(xi, es, dims, n0, nsteps)
+    assert rel_err(f_gpu, f_ref) <= TOL
+    # n / psi of the start of the last step, as the reference's arrays hold
+    assert rel_err(n.D2H(), n_ref) <= TOL
+    assert rel_err(psi.D2H(), psi_ref) <= TOL
+    # ComputeMomentsMeta on the final state
+    from idmx.IdpyCode import IdpyMemory
+    n2 = IdpyMemory.Zeros(eng.V, np.float64, tenet)
+    p2 = IdpyMemory.Zeros(eng.V, np.float64, tenet)
+    u2 = IdpyMemory.Zeros(eng.V * eng.D, np.float64, tenet)
+    eng.moments(pop, n2, p2, u2)
+    assert rel_err(n2.D2H(), mom.n) <= TOL
+    assert rel_err(p2.D2H(), mom.psi) <= TOL
+    assert rel_err(u2.D2H(), mom.u) <= 1e-11
+    eng.close()
+
+
+@pytest.mark.parametrize('xi,es,dims', [('D2Q9', 'D2E4', (33, 31)),
+                                        ('D3Q19', 'D3E4', (12, 13, 10))])
+def test_mass_conserved_to_roundoff(tenet, xi, es, dims):
+    from idmx.LBM._engine import SCEngine
+    n0 = droplet_field(dims, seed=7)
+    eng = SCEngine(tenet, xi, es, dims, 'exp', G, 1.0)
+    pop, swap, n, psi, u = _device_fields(tenet, eng, n0)
+    m0, j0 = eng.mass_momentum(pop)
+    if eng.run(pop, swap, 500):
+        pop, swap = swap, pop
+    m1, j1 = eng.mass_momentum(pop)
+    assert abs(m1 - m0) / m0 < 5e-14
+    # the Shan-Chen force sums to zero on a periodic lattice: momentum too
+    assert np.abs(j1 - j0).max() < 1e-11 * m0
+    eng.close()
+
+
+@pytest.mark.parametrize('xi,es,dims', [('D2Q9', 'D2E4', (24, 21)),
+                                        ('D3Q19', 'D3E4', (11, 10, 9))])
+def test_mrt_matches_oracle(tenet, xi, es, dims):
+    """MRT with a diagonal S of distinct rates: HIP tables from the package's
+    rational Hermite basis, oracle matrices from its own sympy derivation."""
+    from idmx.LBM import hermite
+    from idmx.LBM._engine import SCEngine
+    from oracle import hermite_mrt
+    Q = 9 if xi == 'D2Q9' else 19
+    rng = np.random.default_rng(3)
+    s = rng.uniform(0.6, 1.6, size=Q)
+    S = np.diag(s)
+    n0 = droplet_field(dims, seed=11)
+    eng = SCEngine(tenet, xi, es, dims, 'exp', G, 1.0, mrt=hermite.mrt_tables(xi, S))
+    pop, swap, n, psi, u = _device_fields(tenet, eng, n0)
+    if eng.run(pop, swap, 30):
+        pop, swap = swap, pop
+    import sympy as sp
+    Ssym = sp.diag(*[sp.Rational(v).limit_denominator(10 ** 15) for v in s])
+    mats = hermite_mrt.mrt_matrices(xi, Ssym)
+    f_ref, _, _, _ = _oracle(xi, es, dims, n0, 30, mrt=mats)
+    assert rel_err(eng.pop_to_host_soa(pop).ravel(), f_ref) <= 1e-11
+    eng.close()
+
+
+def test_fp32_shifted_close_to_fp64(tenet):
+    from idmx.LBM._engine import SCEngine
+    dims = (64, 48)
+    n0 = droplet_field(dims, seed=5)
+    out = {}
+    for prec in ('fp64', 'fp32s'):
+        eng = SCEngine(tenet, 'D2Q9', 'D2E4', dims, 'exp', G, 1.0, precision=prec,
+                       rho0=1.0)
+        pop, swap, n, psi, u = _device_fields(tenet, eng, n0)
+        if eng.run(pop, swap, 100):
+            pop, swap = swap, pop
+        out[prec] = eng.pop_to_host_soa(pop)
+        eng.close()
+    assert rel_err(out['fp32s'], out['fp64']) < 2e-5
