@@ -1,0 +1,385 @@
+// batch.hip -- batched Shan-Chen sweep over many D2Q9 domains (C ABI in
+// include/idmx.h, idmx_batch_*). The library owns the packed population
+// buffers; a step is one launch of k_batched over the live work items, and
+// idmx_batch_run replays a captured graph of an even number of such launches.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "batch_kernels.h"
+#include "internal.h"
+
+using namespace idmx;
+
+struct idmx_batch_s {
+  idmx_ctx_s* ctx = nullptr;
+  int prec = 0, estencil = 4;
+  int tx = 128, H = 32;
+  int ndom = 0;
+  std::vector<BatchDom> doms;
+  std::vector<int> active;
+  size_t elem = 8;
+  int64_t pop_elems = 0, sites = 0;
+  void* pop[2] = {nullptr, nullptr};
+  int cur = 0;  // index of the buffer holding the state
+  double* n_prev = nullptr;
+  double* stage = nullptr;  // per-domain FP64 staging (max V)
+  int64_t max_v = 0;
+  BatchDom* d_doms = nullptr;
+  BatchItem* d_items = nullptr;
+  int* d_nitems = nullptr;
+  unsigned long long* d_dmax = nullptr;
+  int max_items = 0, n_items = 0;
+  int grid = 0;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+  double last_ms = 0;
+};
+
+namespace {
+
+template <class ES, class T, bool SHIFT, int TX>
+hipError_t launch_step(idmx_batch_s* b, const void* in, void* out) {
+  hipLaunchKernelGGL((k_batched<ES, T, SHIFT, TX>), dim3(b->grid), dim3(TX), 0, b->ctx->stream,
+                     (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems);
+  return hipGetLastError();
+}
+
+template <class ES, class T, bool SHIFT>
+hipError_t step_tx(idmx_batch_s* b, const void* in, void* out) {
+  switch (b->tx) {
+    case 64: return launch_step<ES, T, SHIFT, 64>(b, in, out);
+    case 256: return launch_step<ES, T, SHIFT, 256>(b, in, out);
+    default: return launch_step<ES, T, SHIFT, 128>(b, in, out);
+  }
+}
+
+template <class T, bool SHIFT>
+hipError_t step_es(idmx_batch_s* b, const void* in, void* out) {
+  switch (b->estencil) {
+    case IDMX_E6: return step_tx<D2E6, T, SHIFT>(b, in, out);
+    case IDMX_E8: return step_tx<D2E8, T, SHIFT>(b, in, out);
+    default: return step_tx<D2E4, T, SHIFT>(b, in, out);
+  }
+}
+
+hipError_t one_step(idmx_batch_s* b, const void* in, void* out) {
+  return b->prec == IDMX_PREC_F64 ? step_es<double, false>(b, in, out)
+                                  : step_es<float, true>(b, in, out);
+}
+
+template <class T, bool SHIFT, int TX>
+hipError_t launch_dn(idmx_batch_s* b) {
+  hipLaunchKernelGGL((k_batched_dn<T, SHIFT, TX>), dim3(b->grid), dim3(TX), 0, b->ctx->stream,
+                     (const T*)b->pop[b->cur], b->d_doms, b->d_items, b->d_nitems, b->n_prev,
+                     b->d_dmax);
+  return hipGetLastError();
+}
+
+template <class T, bool SHIFT>
+hipError_t dn_tx(idmx_batch_s* b) {
+  switch (b->tx) {
+    case 64: return launch_dn<T, SHIFT, 64>(b);
+    case 256: return launch_dn<T, SHIFT, 256>(b);
+    default: return launch_dn<T, SHIFT, 128>(b);
+  }
+}
+
+void drop_graph(idmx_batch_s* b) {
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graph_steps = 0;
+}
+
+// Rebuilds the work-item list from the active flags: for every live domain,
+// x-chunks of TX sites and runs of H rows.
+int rebuild_items(idmx_batch_s* b) {
+  std::vector<BatchItem> items;
+  for (int d = 0; d < b->ndom; d++) {
+    if (!b->active[d]) continue;
+    const BatchDom& D = b->doms[d];
+    for (int m0 = 0; m0 < D.L1; m0 += b->H)
+      for (int x0 = 0; x0 < D.L0; x0 += b->tx)
+        items.push_back({d, x0, m0, std::min(b->H, D.L1 - m0)});
+  }
+  if ((int)items.size() > b->max_items) {
+    if (b->d_items) hipFree(b->d_items);
+    b->max_items = std::max<int>((int)items.size(), 1);
+    IDMX_HIP(hipMalloc(&b->d_items, sizeof(BatchItem) * b->max_items));
+    drop_graph(b);  // the graph holds the old item pointer
+  }
+  b->n_items = (int)items.size();
+  if (!items.empty())
+    IDMX_HIP(hipMemcpyAsync(b->d_items, items.data(), sizeof(BatchItem) * items.size(),
+                            hipMemcpyHostToDevice, b->ctx->stream));
+  IDMX_HIP(hipMemcpyAsync(b->d_nitems, &b->n_items, sizeof(int), hipMemcpyHostToDevice,
+                          b->ctx->stream));
+  IDMX_HIP(hipStreamSynchronize(b->ctx->stream));
+  return IDMX_OK;
+}
+
+}  // namespace
+
+#define BCHECK(cond, msg)     \
+  do {                        \
+    if (!(cond)) {            \
+      idmx::set_error(msg);   \
+      return IDMX_E_ARG;      \
+    }                         \
+  } while (0)
+
+extern "C" {
+
+int idmx_batch_create(idmx_ctx_t ctx, const idmx_batch_domain* doms, int ndom, int precision,
+                      int estencil, int psi_kind, idmx_batch_t* out) {
+  BCHECK(ctx && doms && ndom > 0 && out, "idmx_batch_create: invalid argument");
+  BCHECK(precision == IDMX_PREC_F64 || precision == IDMX_PREC_F32S,
+         "idmx_batch_create: unknown precision");
+  BCHECK(estencil == IDMX_E4 || estencil == IDMX_E6 || estencil == IDMX_E8,
+         "idmx_batch_create: unknown E-stencil");
+  BCHECK(psi_kind >= 0 && psi_kind <= 2, "idmx_batch_create: unknown psi kind");
+  const int R = estencil == IDMX_E4 ? 1 : 2;
+  idmx_batch_s* b = new idmx_batch_s();
+  b->ctx = ctx;
+  b->prec = precision;
+  b->estencil = estencil;
+  b->ndom = ndom;
+  b->elem = precision == IDMX_PREC_F64 ? 8 : 4;
+  int64_t po = 0, so = 0;
+  for (int d = 0; d < ndom; d++) {
+    const idmx_batch_domain& s = doms[d];
+    if (s.L0 < 2 * R + 1 || s.L1 < 2 * R + 1 || s.L0 > 65536 || s.L1 > 65536 || s.tau <= 0) {
+      delete b;
+      set_error("idmx_batch_create: domain sizes must be in [2R+1, 65536], tau > 0");
+      return IDMX_E_ARG;
+    }
+    BatchDom D{};
+    D.L0 = s.L0;
+    D.L1 = s.L1;
+    D.pop_off = po;
+    D.site_off = so;
+    D.G = s.SC_G;
+    D.omega = 1.0 / s.tau;
+    D.rho0 = precision == IDMX_PREC_F32S ? s.rho0 : 0.0;
+    D.psi_kind = psi_kind;
+    const int64_t V = (int64_t)s.L0 * s.L1;
+    po += 9 * V;
+    so += V;
+    b->max_v = std::max(b->max_v, V);
+    b->doms.push_back(D);
+  }
+  b->pop_elems = po;
+  b->sites = so;
+  b->active.assign(ndom, 1);
+  hipError_t e = hipMalloc(&b->pop[0], b->elem * po);
+  if (e == hipSuccess) e = hipMalloc(&b->pop[1], b->elem * po);
+  if (e == hipSuccess) e = hipMalloc(&b->n_prev, sizeof(double) * so);
+  if (e == hipSuccess) e = hipMalloc(&b->stage, sizeof(double) * b->max_v);
+  if (e == hipSuccess) e = hipMalloc(&b->d_doms, sizeof(BatchDom) * ndom);
+  if (e == hipSuccess) e = hipMalloc(&b->d_nitems, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&b->d_dmax, sizeof(unsigned long long) * ndom);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_doms, b->doms.data(), sizeof(BatchDom) * ndom, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(b->n_prev, 0, sizeof(double) * so);
+  if (e != hipSuccess) {
+    idmx_batch_destroy(b);
+    return hip_fail(e, "idmx_batch_create");
+  }
+  hipDeviceProp_t p;
+  hipGetDeviceProperties(&p, ctx->device);
+  b->grid = p.multiProcessorCount * 8;
+  int rc = rebuild_items(b);
+  if (rc) {
+    idmx_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return IDMX_OK;
+}
+
+int idmx_batch_destroy(idmx_batch_t b) {
+  if (!b) return IDMX_OK;
+  if (b->ctx && b->ctx->stream) hipStreamSynchronize(b->ctx->stream);
+  drop_graph(b);
+  for (void* p : {b->pop[0], b->pop[1], (void*)b->n_prev, (void*)b->stage, (void*)b->d_doms,
+                  (void*)b->d_items, (void*)b->d_nitems, (void*)b->d_dmax})
+    if (p) hipFree(p);
+  delete b;
+  return IDMX_OK;
+}
+
+int idmx_batch_set_launch(idmx_batch_t b, int tx, int H, int grid) {
+  BCHECK(b, "idmx_batch_set_launch: null batch");
+  BCHECK(tx == 64 || tx == 128 || tx == 256, "idmx_batch_set_launch: tx is 64, 128 or 256");
+  BCHECK(H >= 1, "idmx_batch_set_launch: H >= 1");
+  BCHECK(grid >= 0, "idmx_batch_set_launch: grid >= 0");
+  b->tx = tx;
+  b->H = H;
+  if (grid > 0) b->grid = grid;
+  drop_graph(b);
+  return rebuild_items(b);
+}
+
+int idmx_batch_get_launch(idmx_batch_t b, int* tx, int* H, int* grid, int* n_items) {
+  BCHECK(b && tx && H && grid && n_items, "idmx_batch_get_launch: invalid argument");
+  *tx = b->tx;
+  *H = b->H;
+  *grid = b->grid;
+  *n_items = b->n_items;
+  return IDMX_OK;
+}
+
+int idmx_batch_set_active(idmx_batch_t b, const int* active) {
+  BCHECK(b && active, "idmx_batch_set_active: invalid argument");
+  for (int d = 0; d < b->ndom; d++) b->active[d] = active[d] ? 1 : 0;
+  return rebuild_items(b);
+}
+
+int idmx_batch_set_density(idmx_batch_t b, int dom, const double* n_host) {
+  BCHECK(b && n_host && dom >= 0 && dom < b->ndom, "idmx_batch_set_density: invalid argument");
+  const BatchDom& D = b->doms[dom];
+  const int64_t V = (int64_t)D.L0 * D.L1;
+  IDMX_HIP(hipMemcpyAsync(b->stage, n_host, sizeof(double) * V, hipMemcpyHostToDevice,
+                          b->ctx->stream));
+  const unsigned grid = (unsigned)std::min<int64_t>((V + 255) / 256, 4096);
+  if (b->prec == IDMX_PREC_F64)
+    hipLaunchKernelGGL((k_batched_init<double, false>), dim3(grid), dim3(256), 0, b->ctx->stream,
+                       (double*)b->pop[b->cur] + D.pop_off, b->stage, V, 0.0);
+  else
+    hipLaunchKernelGGL((k_batched_init<float, true>), dim3(grid), dim3(256), 0, b->ctx->stream,
+                       (float*)b->pop[b->cur] + D.pop_off, b->stage, V, D.rho0);
+  IDMX_HIP(hipGetLastError());
+  IDMX_HIP(hipMemcpyAsync(b->n_prev + D.site_off, b->stage, sizeof(double) * V,
+                          hipMemcpyDeviceToDevice, b->ctx->stream));
+  IDMX_HIP(hipStreamSynchronize(b->ctx->stream));
+  return IDMX_OK;
+}
+
+int idmx_batch_get_density(idmx_batch_t b, int dom, double* n_host) {
+  BCHECK(b && n_host && dom >= 0 && dom < b->ndom, "idmx_batch_get_density: invalid argument");
+  const BatchDom& D = b->doms[dom];
+  const int64_t V = (int64_t)D.L0 * D.L1;
+  const unsigned grid = (unsigned)std::min<int64_t>((V + 255) / 256, 4096);
+  if (b->prec == IDMX_PREC_F64)
+    hipLaunchKernelGGL((k_batched_density<double, false>), dim3(grid), dim3(256), 0,
+                       b->ctx->stream, (const double*)b->pop[b->cur] + D.pop_off, b->stage, V,
+                       0.0);
+  else
+    hipLaunchKernelGGL((k_batched_density<float, true>), dim3(grid), dim3(256), 0,
+                       b->ctx->stream, (const float*)b->pop[b->cur] + D.pop_off, b->stage, V,
+                       D.rho0);
+  IDMX_HIP(hipGetLastError());
+  IDMX_HIP(hipMemcpyAsync(n_host, b->stage, sizeof(double) * V, hipMemcpyDeviceToHost,
+                          b->ctx->stream));
+  IDMX_HIP(hipStreamSynchronize(b->ctx->stream));
+  return IDMX_OK;
+}
+
+int idmx_batch_run(idmx_batch_t b, int nsteps) {
+  BCHECK(b && nsteps >= 0, "idmx_batch_run: invalid argument");
+  hipStream_t st = b->ctx->stream;
+  IDMX_HIP(hipEventRecord(b->ctx->ev0, st));
+  if (b->n_items > 0 && nsteps > 0) {
+    // graph of an even number of steps (A->B, B->A, ...): replays keep the
+    // state in pop[cur]
+    const int G = 2 * std::max(1, std::min(nsteps / 2, 50));
+    if (nsteps >= G && b->graph_steps != G) {
+      drop_graph(b);
+      hipGraph_t g;
+      IDMX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < G; k++) {
+        const int src = (b->cur + k) & 1;
+        hipError_t e = one_step(b, b->pop[src], b->pop[src ^ 1]);
+        if (e != hipSuccess) {
+          hipStreamEndCapture(st, &g);
+          return hip_fail(e, "idmx_batch_run capture");
+        }
+      }
+      IDMX_HIP(hipStreamEndCapture(st, &g));
+      IDMX_HIP(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+      hipGraphDestroy(g);
+      b->graph_steps = G;
+    }
+    int done = 0;
+    // the graph was captured for the parity the state had then; cur is
+    // unchanged after every replay, so it matches
+    while (b->graph && nsteps - done >= b->graph_steps) {
+      IDMX_HIP(hipGraphLaunch(b->graph, st));
+      done += b->graph_steps;
+    }
+    for (; done < nsteps; done++) {
+      hipError_t e = one_step(b, b->pop[b->cur], b->pop[b->cur ^ 1]);
+      if (e != hipSuccess) return hip_fail(e, "idmx_batch_run launch");
+      b->cur ^= 1;
+      drop_graph(b);  // parity changed: the captured buffers no longer match
+    }
+  }
+  IDMX_HIP(hipEventRecord(b->ctx->ev1, st));
+  IDMX_HIP(hipEventSynchronize(b->ctx->ev1));
+  float ms = 0;
+  IDMX_HIP(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+  b->last_ms = ms;
+  return IDMX_OK;
+}
+
+int idmx_batch_check(idmx_batch_t b, double* dmax) {
+  BCHECK(b && dmax, "idmx_batch_check: invalid argument");
+  hipStream_t st = b->ctx->stream;
+  IDMX_HIP(hipMemsetAsync(b->d_dmax, 0, sizeof(unsigned long long) * b->ndom, st));
+  if (b->n_items > 0) {
+    hipError_t e = b->prec == IDMX_PREC_F64 ? dn_tx<double, false>(b) : dn_tx<float, true>(b);
+    if (e != hipSuccess) return hip_fail(e, "idmx_batch_check");
+  }
+  std::vector<unsigned long long> h(b->ndom);
+  IDMX_HIP(hipMemcpyAsync(h.data(), b->d_dmax, sizeof(unsigned long long) * b->ndom,
+                          hipMemcpyDeviceToHost, st));
+  IDMX_HIP(hipStreamSynchronize(st));
+  for (int d = 0; d < b->ndom; d++) {
+    double v;
+    memcpy(&v, &h[d], sizeof(double));
+    dmax[d] = b->active[d] ? v : 0.0;
+  }
+  return IDMX_OK;
+}
+
+int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src) {
+  BCHECK(dst && src && dst->ndom == src->ndom, "idmx_batch_copy_state: batches differ");
+  for (int d = 0; d < dst->ndom; d++)
+    BCHECK(dst->doms[d].L0 == src->doms[d].L0 && dst->doms[d].L1 == src->doms[d].L1,
+           "idmx_batch_copy_state: domain sizes differ");
+  IDMX_HIP(hipStreamSynchronize(src->ctx->stream));
+  hipStream_t st = dst->ctx->stream;
+  for (int d = 0; d < dst->ndom; d++) {
+    const BatchDom& S = src->doms[d];
+    const BatchDom& D = dst->doms[d];
+    const int64_t V = (int64_t)S.L0 * S.L1;
+    const unsigned grid = (unsigned)std::min<int64_t>((V + 255) / 256, 4096);
+    const void* in = (const char*)src->pop[src->cur] + src->elem * S.pop_off;
+    void* out = (char*)dst->pop[dst->cur] + dst->elem * D.pop_off;
+    if (src->prec == IDMX_PREC_F32S && dst->prec == IDMX_PREC_F64)
+      hipLaunchKernelGGL((k_batched_convert<float, true, double, false>), dim3(grid), dim3(256),
+                         0, st, (const float*)in, (double*)out, V, S.rho0, 0.0);
+    else if (src->prec == IDMX_PREC_F64 && dst->prec == IDMX_PREC_F32S)
+      hipLaunchKernelGGL((k_batched_convert<double, false, float, true>), dim3(grid), dim3(256),
+                         0, st, (const double*)in, (float*)out, V, 0.0, D.rho0);
+    else if (src->prec == IDMX_PREC_F64)
+      hipLaunchKernelGGL((k_batched_convert<double, false, double, false>), dim3(grid),
+                         dim3(256), 0, st, (const double*)in, (double*)out, V, 0.0, 0.0);
+    else
+      hipLaunchKernelGGL((k_batched_convert<float, true, float, true>), dim3(grid), dim3(256), 0,
+                         st, (const float*)in, (float*)out, V, S.rho0, D.rho0);
+    IDMX_HIP(hipGetLastError());
+    IDMX_HIP(hipMemcpyAsync(dst->n_prev + D.site_off, src->n_prev + S.site_off,
+                            sizeof(double) * V, hipMemcpyDeviceToDevice, st));
+  }
+  IDMX_HIP(hipStreamSynchronize(st));
+  return IDMX_OK;
+}
+
+int idmx_batch_last_run_ms(idmx_batch_t b, double* ms) {
+  BCHECK(b && ms, "idmx_batch_last_run_ms: invalid argument");
+  *ms = b->last_ms;
+  return IDMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+// batch_kernels.h -- one launch advances many independent D2Q9 Shan-Chen
+// domains of different sizes and couplings (a (G, R, L) parameter sweep).
+//
+// The reference bakes V, L_i and STR_i into every kernel as #defines
+// (idpy/LBM/LBM.py:542-569, idpy/IdpyCode/IdpyCode.py:770-781), so each domain
+// size is a new compile and a new launch. Here the geometry is data: a table
+// of domain descriptors and a list of work items (domain, x-chunk of TX
+// sites, run of H rows). Persistent blocks stride over the item list, each
+// item is one march of the fused step (the same algorithm as the
+// single-domain kernel: LDS psi ring with halo, populations kept in
+// registers, push streaming). The number of live items is read from device
+// memory, so the host can drop converged domains between graph replays
+// without re-capturing the graph.
+#pragma once
+#include "sc_kernels.h"
+
+namespace idmx {
+
+struct BatchDom {
+  int L0, L1;
+  int64_t pop_off;   // element offset of the domain's Q*V populations
+  int64_t site_off;  // offset of the domain's V sites in per-site arrays
+  double G, omega, rho0;
+  int psi_kind;
+  int pad;
+};
+
+struct BatchItem {
+  int dom, x0, m0, Hs;
+};
+
+// One TX-wide, Hs-row march of the 2D fused step on one domain (R = stencil
+// reach). Same structure as k_fused's 2D path; see there for the ring and
+// register-history logic.
+template <class ES, class T, bool SHIFT, int TX>
+__device__ inline void march2d(const T* __restrict__ pop, T* __restrict__ pop_swap,
+                               const Geom& geo, const Phys<T>& P, int x0, int m0,
+                               int Hs, T (*lds)[TX + 2 * ES::R]) {
+  using S = D2Q9;
+  constexpr int Q = S::Q, R = ES::R;
+  constexpr int NS = 2 * R + 2;
+  const int Lm = geo.L1;
+  const int64_t PS = geo.L0;
+  const int tid = threadIdx.x;
+  const int x = x0 + tid;
+  const bool own_valid = x < geo.L0;
+  const int64_t own_off = x % geo.L0;
+  const bool has_halo = tid < 2 * R;
+  const int hx = tid < R ? tid : TX + tid;
+  const int64_t halo_off = has_halo ? ((x0 - R + hx) % geo.L0 + geo.L0) % geo.L0 : 0;
+
+  T fh[R + 1][Q];
+  T dnh[R + 1];
+  for (int j = 0; j < Hs + 2 * R; j++) {
+    const int mj = ((m0 - R + j) % Lm + Lm) % Lm;
+    const int64_t base = (int64_t)mj * PS;
+    const int slot = j % NS;
+    {
+      T s = 0;
+#pragma unroll
+      for (int q = 0; q < Q; q++) fh[R][q] = pop[(size_t)q * geo.V + base + own_off];
+#pragma unroll
+      for (int q = 0; q < Q; q++) s += fh[R][q];
+      dnh[R] = s;
+      lds[slot][R + tid] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+    }
+    if (has_halo) {
+      T s = 0;
+#pragma unroll
+      for (int q = 0; q < Q; q++) s += pop[(size_t)q * geo.V + base + halo_off];
+      lds[slot][hx] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+    }
+    __syncthreads();
+    if (j >= 2 * R && own_valid) {
+      int sl[2 * R + 1];
+#pragma unroll
+      for (int k = 0; k <= 2 * R; k++) sl[k] = (j - 2 * R + k) % NS;
+      const int cx = R + tid;
+      T gx = 0, gy = 0, gz = 0;
+#pragma unroll
+      for (int e = 0; e < ES::QE; e++) {
+        const T v = lds[sl[R + ES::ey[e]]][cx + ES::ex[e]];
+        const T w = T(ES::w[e]);
+        if (ES::ex[e] != 0) gx += T(ES::ex[e]) * w * v;
+        if (ES::ey[e] != 0) gy += T(ES::ey[e]) * w * v;
+      }
+      const T lpsi = lds[sl[R]][cx];
+      T fo[Q], u[3];
+      collide<S, T, SHIFT, 0>(fh[0], dnh[0], gx, gy, gz, lpsi, P, fo, u);
+      push_store<S, T>(pop_swap, geo, x, m0 + j - 2 * R, 0, fo);
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      dnh[k] = dnh[k + 1];
+#pragma unroll
+      for (int q = 0; q < Q; q++) fh[k][q] = fh[k + 1][q];
+    }
+  }
+}
+
+template <class T>
+__device__ inline Phys<T> dom_phys(const BatchDom& d) {
+  Phys<T> P;
+  P.G = (T)d.G;
+  P.omega = (T)d.omega;
+  P.rho0 = (T)d.rho0;
+  P.psi_kind = d.psi_kind;
+  P.mrtA = P.mrtB = P.mrtG = nullptr;
+  return P;
+}
+
+template <class ES, class T, bool SHIFT, int TX>
+__global__ void __launch_bounds__(TX)
+k_batched(const T* __restrict__ pop_all, T* __restrict__ swap_all,
+          const BatchDom* __restrict__ doms, const BatchItem* __restrict__ items,
+          const int* __restrict__ n_items) {
+  __shared__ T lds[2 * ES::R + 2][TX + 2 * ES::R];
+  const int ni = *n_items;
+  for (int it = blockIdx.x; it < ni; it += gridDim.x) {
+    const BatchItem w = items[it];
+    const BatchDom d = doms[w.dom];
+    const Geom geo{d.L0, d.L1, 1, (int64_t)d.L0 * d.L1};
+    const Phys<T> P = dom_phys<T>(d);
+    march2d<ES, T, SHIFT, TX>(pop_all + d.pop_off, swap_all + d.pop_off, geo, P, w.x0,
+                              w.m0, w.Hs, lds);
+    __syncthreads();
+  }
+}
+
+// Per-domain max |n - n_prev| over the live items (FP64), n_prev <- n.
+// dmax holds the bit patterns of non-negative doubles, whose unsigned order
+// is their numeric order, so a 64-bit atomic max reduces them.
+template <class T, bool SHIFT, int TX>
+__global__ void __launch_bounds__(TX)
+k_batched_dn(const T* __restrict__ pop_all, const BatchDom* __restrict__ doms,
+             const BatchItem* __restrict__ items, const int* __restrict__ n_items,
+             double* __restrict__ n_prev, unsigned long long* __restrict__ dmax) {
+  const int ni = *n_items;
+  for (int it = blockIdx.x; it < ni; it += gridDim.x) {
+    const BatchItem w = items[it];
+    const BatchDom d = doms[w.dom];
+    const int64_t V = (int64_t)d.L0 * d.L1;
+    const T* pop = pop_all + d.pop_off;
+    double m = 0.0;
+    const int x = w.x0 + threadIdx.x;
+    if (x < d.L0) {
+      for (int r = 0; r < w.Hs; r++) {
+        const int64_t g = (int64_t)(w.m0 + r) * d.L0 + x;
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; q++) s += (double)pop[(size_t)q * V + g];
+        const double n = SHIFT ? s + d.rho0 : s;
+        double* np = n_prev + d.site_off + g;
+        m = fmax(m, fabs(n - *np));
+        *np = n;
+      }
+    }
+    // block max (TX threads)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    __shared__ double part[TX / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double r = part[0];
+      for (int k = 1; k < TX / 64; k++) r = fmax(r, part[k]);
+      atomicMax(dmax + w.dom, (unsigned long long)__double_as_longlong(r));
+    }
+    __syncthreads();
+  }
+}
+
+// f = f_eq(n, 0) (shifted when SHIFT) for one domain, n in FP64.
+template <class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_batched_init(T* __restrict__ pop, const double* __restrict__ n, int64_t V, double rho0) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      double f = D2Q9::w[q] * n[g];
+      if (SHIFT) f -= D2Q9::w[q] * rho0;
+      pop[(size_t)q * V + g] = (T)f;
+    }
+  }
+}
+
+// n = sum f (+ rho0) in FP64 for one domain.
+template <class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_batched_density(const T* __restrict__ pop, double* __restrict__ n, int64_t V, double rho0) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) s += (double)pop[(size_t)q * V + g];
+    n[g] = SHIFT ? s + rho0 : s;
+  }
+}
+
+// Population copy between two batches of the same domains (e.g. FP32
+// shifted -> FP64), unshifting / reshifting on the way.
+template <class TI, bool SI, class TO, bool SO>
+__global__ void __launch_bounds__(256)
+k_batched_convert(const TI* __restrict__ src, TO* __restrict__ dst, int64_t V, double rho_in,
+                  double rho_out) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      double f = (double)src[(size_t)q * V + g];
+      if (SI) f += D2Q9::w[q] * rho_in;
+      if (SO) f -= D2Q9::w[q] * rho_out;
+      dst[(size_t)q * V + g] = (TO)f;
+    }
+  }
+}
+
+}  // namespace idmx

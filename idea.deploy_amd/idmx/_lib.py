@@ -26,6 +26,14 @@ class ScDesc(C.Structure):
                 ('mrt_A', C.c_void_p), ('mrt_beta', C.c_void_p), ('mrt_gamma', C.c_void_p)]
 
 
+class BatchDomain(C.Structure):
+    """Mirror of idmx_batch_domain."""
+    _fields_ = [('L0', C.c_int), ('L1', C.c_int), ('SC_G', C.c_double), ('tau', C.c_double),
+                ('rho0', C.c_double)]
+
+
+c_int_p = C.POINTER(C.c_int)
+
 # (name, restype, argtypes)
 _SIGS = [
     ('idmx_abi_version', C.c_int, []),
@@ -67,6 +75,31 @@ _SIGS = [
     ('idmx_sc_pop_to_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ('idmx_sc_pop_from_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ('idmx_sc_last_run_ms', C.c_int, [C.c_void_p, c_double_p]),
+    ('idmx_batch_create', C.c_int, [C.c_void_p, C.POINTER(BatchDomain), C.c_int, C.c_int,
+                                    C.c_int, C.c_int, c_void_pp]),
+    ('idmx_batch_destroy', C.c_int, [C.c_void_p]),
+    ('idmx_batch_set_launch', C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    ('idmx_batch_get_launch', C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p, c_int_p]),
+    ('idmx_batch_set_active', C.c_int, [C.c_void_p, c_int_p]),
+    ('idmx_batch_set_density', C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ('idmx_batch_get_density', C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ('idmx_batch_run', C.c_int, [C.c_void_p, C.c_int]),
+    ('idmx_batch_check', C.c_int, [C.c_void_p, C.c_void_p]),
+    ('idmx_batch_copy_state', C.c_int, [C.c_void_p, C.c_void_p]),
+    ('idmx_batch_last_run_ms', C.c_int, [C.c_void_p, c_double_p]),
+    ('idmx_nn_conv3x3_fwd', C.c_int, [C.c_void_p] * 6 + [C.c_int] * 6),
+    ('idmx_nn_conv3x3_bwd_data', C.c_int, [C.c_void_p] * 5 + [C.c_int] * 5),
+    ('idmx_nn_conv3x3_bwd_weight', C.c_int, [C.c_void_p] * 5 + [C.c_int] * 5),
+    ('idmx_nn_relu_bwd', C.c_int, [C.c_void_p] * 4 + [C.c_int64]),
+    ('idmx_nn_pool2', C.c_int, [C.c_void_p] * 3 + [C.c_int] * 5),
+    ('idmx_nn_up2', C.c_int, [C.c_void_p] * 3 + [C.c_int] * 7),
+    ('idmx_nn_copy_channels', C.c_int, [C.c_void_p] * 3 + [C.c_int64] + [C.c_int] * 5),
+    ('idmx_nn_add', C.c_int, [C.c_void_p] * 4 + [C.c_int64]),
+    ('idmx_nn_mse', C.c_int, [C.c_void_p] * 4 + [C.c_int64, c_double_p]),
+    ('idmx_nn_adam', C.c_int, [C.c_void_p] * 6 + [C.c_int64] + [C.c_float] * 4 + [C.c_int]),
+    ('idmx_nn_cast', C.c_int, [C.c_void_p] * 3 + [C.c_int64, C.c_int]),
+    ('idmx_nn_resize_to_lattice', C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                            C.c_int, C.c_double]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

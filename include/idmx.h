@@ -196,6 +196,102 @@ int idmx_sc_get_launch(idmx_sc_t sim, int* tx, int* ty, int* H);
  * on the simulation's stream around its kernels. */
 int idmx_sc_last_run_ms(idmx_sc_t sim, double* ms);
 
+/* ---- batched sweep over many D2Q9 domains ----------------------------
+ * No single reference counterpart: the reference runs one
+ * ShanChenMultiPhase per (G, R, L) point and compiles every domain size into
+ * its own kernel (sizes are #defines, idpy/LBM/LBM.py:542-569). Here one
+ * launch advances all live domains of a sweep (BGK, tau per domain, periodic,
+ * centred or arbitrary initial densities), with populations packed in
+ * library-owned HBM buffers. */
+typedef struct idmx_batch_s* idmx_batch_t;
+typedef struct {
+  int L0, L1;        /* domain size                                      */
+  double SC_G;       /* coupling                                         */
+  double tau;        /* BGK relaxation time                              */
+  double rho0;       /* population shift (IDMX_PREC_F32S)                */
+} idmx_batch_domain;
+
+int idmx_batch_create(idmx_ctx_t ctx, const idmx_batch_domain* doms, int ndom,
+                      int precision, int estencil, int psi_kind,
+                      idmx_batch_t* out);
+int idmx_batch_destroy(idmx_batch_t b);
+/* Work decomposition: x-chunks of tx sites (64, 128, 256), runs of H rows,
+ * grid persistent blocks (0 keeps the current grid). Picked by the
+ * autotuner (idmx.tune). */
+int idmx_batch_set_launch(idmx_batch_t b, int tx, int H, int grid);
+int idmx_batch_get_launch(idmx_batch_t b, int* tx, int* H, int* grid,
+                          int* n_items);
+/* active[d] != 0 keeps domain d in the work list. */
+int idmx_batch_set_active(idmx_batch_t b, const int* active);
+/* f = f_eq(n, 0) for domain dom from a host float64 density (x fastest);
+ * also resets the domain's reference for idmx_batch_check. */
+int idmx_batch_set_density(idmx_batch_t b, int dom, const double* n_host);
+int idmx_batch_get_density(idmx_batch_t b, int dom, double* n_host);
+/* Advances every live domain by nsteps (graph replays of 100 steps where
+ * nsteps allows). */
+int idmx_batch_run(idmx_batch_t b, int nsteps);
+/* dmax[d] = max_x |n(x) - n_prev(x)| over live domain d in FP64, where
+ * n_prev is the density at the previous check (or set_density); 0 for
+ * inactive domains. */
+int idmx_batch_check(idmx_batch_t b, double* dmax);
+/* Copies the state of every domain from src to dst (same domains; FP32
+ * shifted <-> FP64 conversion included). */
+int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src);
+int idmx_batch_last_run_ms(idmx_batch_t b, double* ms);
+
+/* ---- warm-start network kernels ---------------------------------------
+ * No reference counterpart (idpy has no learned component). Tensors are NHWC
+ * on the device; activations and conv weights in bf16 (uint16 bit patterns),
+ * master weights, gradients and losses in float32. A conv weight is
+ * [Cout][9*Cin] with k = (3*ky + kx)*Cin + ci. Cin of a forward conv (and
+ * Cout of its backward-data pass) must be a multiple of 4. */
+/* out = act(conv3x3(in, wt) + bias), zero padding 1; out_f32 (may be NULL)
+ * receives the FP32 pre-activation; relu != 0 applies max(0, .). */
+int idmx_nn_conv3x3_fwd(idmx_ctx_t ctx, const void* in, const void* wt,
+                        const float* bias, void* out, float* out_f32, int N,
+                        int H, int W, int Cin, int Cout, int relu);
+/* dx = conv3x3^T(dy, w); w is the FP32 master weight, wd_scratch holds
+ * 9*Cin*Cout bf16 values. */
+int idmx_nn_conv3x3_bwd_data(idmx_ctx_t ctx, const void* dy, const float* w,
+                             void* wd_scratch, void* dx, int N, int H, int W,
+                             int Cin, int Cout);
+/* dw += sum over pixels of dy (x) im2col(in); db += sum of dy (db may be
+ * NULL). */
+int idmx_nn_conv3x3_bwd_weight(idmx_ctx_t ctx, const void* in, const void* dy,
+                               float* dw, float* db, int N, int H, int W,
+                               int Cin, int Cout);
+int idmx_nn_relu_bwd(idmx_ctx_t ctx, const void* y, const void* dy, void* dx,
+                     int64_t n);
+/* 2x2 average pool (bwd = 0: x is N,H,W,C -> y N,H/2,W/2,C; bwd = 1: x is the
+ * pooled gradient, y the N,H,W,C gradient). */
+int idmx_nn_pool2(idmx_ctx_t ctx, const void* x, void* y, int N, int H, int W,
+                  int C, int bwd);
+/* bwd = 0: nearest 2x upsample of x (N,H,W,C) into channels [c0, c0+C) of
+ * y (N,2H,2W,Cy). bwd = 1: x is a gradient of shape (N,2H,2W,Cy); y
+ * (N,H,W,C) receives the sum over each 2x2 block of its channels
+ * [c0, c0+C). */
+int idmx_nn_up2(idmx_ctx_t ctx, const void* x, void* y, int N, int H, int W,
+                int C, int Cy, int c0, int bwd);
+int idmx_nn_copy_channels(idmx_ctx_t ctx, const void* x, void* y, int64_t P,
+                          int Cx, int cx0, int Cy, int cy0, int C);
+int idmx_nn_add(idmx_ctx_t ctx, const void* a, const void* b, void* y,
+                int64_t n);
+/* loss = mean (pred - tgt)^2, grad = 2 (pred - tgt) / n in bf16. */
+int idmx_nn_mse(idmx_ctx_t ctx, const float* pred, const float* tgt,
+                void* grad, int64_t n, double* loss);
+/* Adam step t >= 1 on FP32 w with gradient g (zeroed afterwards); refreshes
+ * the bf16 copy wb (may be NULL). */
+int idmx_nn_adam(idmx_ctx_t ctx, float* w, float* g, float* m, float* v,
+                 void* wb, int64_t n, float lr, float b1, float b2, float eps,
+                 int t);
+/* float32 <-> bf16 */
+int idmx_nn_cast(idmx_ctx_t ctx, const void* x, void* y, int64_t n,
+                 int to_bf16);
+/* Bilinear resize of an S x S float32 image to an L x L float64 lattice
+ * density, plus a constant. */
+int idmx_nn_resize_to_lattice(idmx_ctx_t ctx, const float* img, int S,
+                              double* n, int L, double add);
+
 #ifdef __cplusplus
 }
 #endif
