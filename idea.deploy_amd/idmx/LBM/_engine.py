@@ -30,6 +30,7 @@ class SCEngine:
         self.precision = precision
         self.rho0 = float(rho0) if precision == 'fp32s' else 0.0
         self.pop_dtype = np.float64 if precision == 'fp64' else np.float32
+        self.is_mrt = mrt is not None
         d = _lib.ScDesc()
         d.xi = XI_IDS[xi_name]
         d.estencil = E_IDS[e_name[2:]]

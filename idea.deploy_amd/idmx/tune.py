@@ -1,0 +1,108 @@
+"""Autotuner for the launch shapes of the fused step and the batched sweep.
+
+The fused step's shape is (tile width, planes per block march); the batched
+sweep's is (x-chunk width TX, rows per item H, persistent grid size). The
+tuner times every candidate on scratch buffers (the caller's state is not
+touched), keeps the fastest and caches it per device and workload signature
+in a JSON file ($IDMX_TUNE_CACHE, default ~/.cache/idmx_tune.json).
+"""
+import json
+import os
+
+import numpy as np
+
+_CACHE = os.environ.get('IDMX_TUNE_CACHE', os.path.join(os.path.expanduser('~'), '.cache',
+                                                         'idmx_tune.json'))
+
+
+def _load():
+    try:
+        return json.load(open(_CACHE))
+    except Exception:
+        return {}
+
+
+def _store(key, val):
+    c = _load()
+    c[key] = val
+    try:
+        os.makedirs(os.path.dirname(_CACHE), exist_ok=True)
+        json.dump(c, open(_CACHE, 'w'), indent=1)
+    except OSError:
+        pass
+
+
+def sim_candidates(D, L_march):
+    tiles = (64, 256) if D == 2 else (32, 64)
+    Hs = [h for h in (8, 16, 32, 64, 128) if h <= L_march] or [L_march]
+    return [(t, h) for t in tiles for h in Hs]
+
+
+def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
+    """Times the fused step of an SCEngine for each (tile, H) candidate on
+    scratch populations initialised from n0 (default: uniform 1.0 plus
+    noise) and sets the fastest on the engine. Returns (best, timings)."""
+    from .IdpyCode import IdpyMemory
+    from .LBM._base import PopArray
+    key = 'sim|%s|%s|%s|%s|%s|%s' % (engine.tenet.GetDeviceName(), engine.xi_name,
+                                      engine.e_name, 'x'.join(map(str, engine.dims)),
+                                      engine.precision, 'mrt' if engine.is_mrt else 'bgk')
+    if use_cache:
+        c = _load().get(key)
+        if c:
+            if apply:
+                engine.set_launch(*c['best'])
+            return tuple(c['best']), c['timings']
+    t = engine.tenet
+    V, D = engine.V, engine.D
+    if n0 is None:
+        n0 = 1.0 + 1e-3 * np.sin(np.arange(V) * 0.37)
+    n = IdpyMemory.OnDevice(np.asarray(n0, dtype=np.float64), t)
+    u = IdpyMemory.Zeros(V * D, np.float64, t)
+    a, b = PopArray(engine, t), PopArray(engine, t)
+    engine.init_pop(a, n, u)
+    L_march = engine.dims[-1]
+    timings = {}
+    old = engine.get_launch()
+    for tile, H in sim_candidates(D, L_march):
+        engine.set_launch(tile, H)
+        engine.run(a, b, 2)
+        engine.run(a, b, steps)
+        timings['%d,%d' % (tile, H)] = engine.last_run_ms() / steps
+    best = min(timings, key=timings.get)
+    best = tuple(int(v) for v in best.split(','))
+    engine.set_launch(*(best if apply else old[::2]))
+    for x in (a, b, n, u):
+        x.Free()
+    _store(key, {'best': best, 'timings': timings})
+    return best, timings
+
+
+def tune_batch(tenet, points, precision, estencil='E4', steps=20, use_cache=True):
+    """Times k_batched for (TX, H, grid) candidates on a scratch batch of the
+    same domains (cold-start droplets) and returns the fastest
+    (tx, H, grid) plus all timings (ms per step)."""
+    from .sweep import Batch, cold_start
+    sig = '%d:%d:%d' % (len(points), sum(p.L * p.L for p in points), max(p.L for p in points))
+    key = 'batch|%s|%s|%s|%s' % (tenet.GetDeviceName(), precision, estencil, sig)
+    if use_cache:
+        c = _load().get(key)
+        if c:
+            return tuple(c['best']), c['timings']
+    b = Batch(tenet, points, precision, estencil)
+    for i, p in enumerate(points):
+        b.set_density(i, cold_start(p))
+    cus = b.get_launch()[2] // 8
+    timings = {}
+    for tx in (64, 128, 256):
+        for H in (16, 32, 64):
+            for g in (cus * 4, cus * 8, cus * 16):
+                b.set_launch(tx, H, g)
+                b.run(2)
+                ms = b.run(steps) / steps
+                timings['%d,%d,%d' % (tx, H, g)] = ms
+    b.close()
+    best = min(timings, key=timings.get)
+    best = tuple(int(v) for v in best.split(','))
+    _store(key, {'best': best, 'timings': timings})
+    return best, timings
