@@ -63,7 +63,7 @@ def workload_desc(name, w):
     return d
 
 
-def run_gpu(name, steps, warmup):
+def run_gpu(name, steps, warmup, tune=True):
     from idmx import configs
     from idmx.IdpyCode import IdpyMemory, Tenet
     from idmx.LBM import hermite
@@ -72,11 +72,14 @@ def run_gpu(name, steps, warmup):
     w = WORKLOADS[name]
     tenet = Tenet(0)
     n0 = configs.sphere3d() if w['init'] == 'sphere3d' else configs.droplets2d()
-    mrt = hermite.mrt_tables(w['xi'], mrt_S(w['xi'])) if w['coll'] == 'mrt' else None
+    mrt = hermite.mrt_kernel_tables(w['xi'], mrt_S(w['xi'])) if w['coll'] == 'mrt' else None
     rho0 = 1.0 if w['prec'] == 'fp32s' else 0.0
     eng = SCEngine(tenet, w['xi'], w['es'], w['dims'], 'exp', configs.SC_G, 1.0,
                    precision=w['prec'], rho0=rho0, mrt=mrt)
     V, D = eng.V, eng.D
+    if tune:
+        from idmx import tune as tuner
+        tuner.tune_sim(eng, steps=5, use_cache=False)
     n = IdpyMemory.OnDevice(n0, tenet)
     u = IdpyMemory.Zeros(V * D, np.float64, tenet)
     pop = PopArray(eng, tenet)
@@ -103,7 +106,8 @@ def run_gpu(name, steps, warmup):
         'kernel_ms': ev_ms / steps,
         'B_alg': B_alg,
         'mass_rel_drift': abs(m1 - m0) / m0,
-        'launch': {'tile_x': launch[0], 'tile_y': launch[1], 'planes_per_block': launch[2]},
+        'launch': {'tile_x': launch[0], 'tile_y': launch[1], 'planes_per_block': launch[2],
+                   'hold_in_registers': bool(launch[3])},
         'device': tenet.GetDeviceName(),
     }
     eng.close()
@@ -145,6 +149,68 @@ def cpu_baseline(name, budget_s=12.0):
                       % (w['xi'], w['es'], w['coll'].upper(), 'x'.join(map(str, dims)), k, dt)}
 
 
+def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
+    """Time to steady state (|dn|_max < 1e-10 between checks 200 steps apart)
+    of a (G, R, L) sweep of centred droplets: plain per-point FP64 runs from
+    the cold start vs batched FP32->FP64 runs from the cold start and from
+    the learned warm start. L uniform in {64, 96, ..., Lmax}, G uniform in
+    [-5.5, -4.2] (textbook normalisation, SC_G = G/3), R in [0.1L, 0.35L].
+    The warm-start net is trained first on equilibrated fields of other
+    points produced by the batched solver."""
+    from idmx import configs, sweep, warmstart
+    from idmx.IdpyCode import Tenet
+    tenet = Tenet(0)
+    Ls = tuple(range(64, Lmax + 1, 32))
+    train_pts = sweep.sample_points(n_train, configs.SEED0 + 5, Ls=Ls)
+    eval_pts = sweep.sample_points(n_eval, configs.SEED0 + 4, Ls=Ls)
+    out = {'n_train': n_train, 'n_eval': n_eval, 'L': [Ls[0], Ls[-1]]}
+    t0 = time.perf_counter()
+    res_tr, f_tr = sweep.run_sweep(tenet, train_pts, mode='fp32+fp64', tol=1e-8)
+    out['train_data_s'] = time.perf_counter() - t0
+    log('training fields: %.1f s, converged %d/%d' % (out['train_data_s'],
+                                                       res_tr.converged.sum(), n_train))
+    X = np.stack([warmstart.input_image(p.G, p.R, p.L)[0] for p in train_pts])
+    Y = np.stack([warmstart.target_image(f, p.L) - warmstart.input_image(p.G, p.R, p.L)[1]
+                  for f, p in zip(f_tr, train_pts)]).astype(np.float32)
+    t0 = time.perf_counter()
+    net, hist = warmstart.train(tenet, X, Y, epochs=epochs, batch=32, log=log)
+    out['train_s'] = time.perf_counter() - t0
+    out['train_loss'] = [float(hist[0]), float(hist[-1])]
+    # held-out image error of the net vs the cold start
+    Xe = np.stack([warmstart.input_image(p.G, p.R, p.L)[0] for p in eval_pts])
+    # plain per-point FP64 runs from the cold start
+    steps_p, conv_p, wall_p, f_p = sweep.plain_runs(tenet, eval_pts)
+    out['plain_fp64'] = {'wall_s': wall_p, 'steps_mean': float(steps_p.mean()),
+                         'converged': int(conv_p.sum())}
+    log('plain: %.1f s' % wall_p)
+    res_c, f_c = sweep.run_sweep(tenet, eval_pts, mode='fp32+fp64')
+    out['batched_cold'] = {'wall_s': res_c.wall_s, 'steps_mean': float(res_c.steps.mean()),
+                           'converged': int(res_c.converged.sum())}
+    log('batched cold: %.1f s' % res_c.wall_s)
+    t0 = time.perf_counter()
+    starts = warmstart.predict_fields(tenet, net, eval_pts)
+    t_inf = time.perf_counter() - t0
+    res_w, f_w = sweep.run_sweep(tenet, eval_pts, start=starts, mode='fp32+fp64')
+    out['batched_warm'] = {'wall_s': res_w.wall_s + t_inf, 'inference_s': t_inf,
+                           'steps_mean': float(res_w.steps.mean()),
+                           'converged': int(res_w.converged.sum()),
+                           'fallbacks': int(res_w.fallback.sum())}
+    log('batched warm: %.1f s' % (res_w.wall_s + t_inf))
+    # initial-field error of warm vs cold start against the converged fields
+    err0 = lambda st, f: float(np.abs(st.ravel() - f.ravel()).max())
+    out['start_err_cold'] = float(np.median([err0(sweep.cold_start(p), f)
+                                             for p, f in zip(eval_pts, f_p)]))
+    out['start_err_warm'] = float(np.median([err0(st, f) for st, f in zip(starts, f_p)]))
+    dp = lambda f, p: sweep.laplace_dp(f, p.G)
+    rel = [abs(dp(fw, p) - dp(fp, p)) / abs(dp(fp, p)) for fw, fp, p in zip(f_w, f_p, eval_pts)]
+    out['laplace_rel_diff_max'] = float(np.max(rel))
+    out['speedup_warm_vs_plain'] = wall_p / out['batched_warm']['wall_s']
+    out['speedup_cold_batched_vs_plain'] = wall_p / res_c.wall_s
+    net.free()
+    tenet.End()
+    return out
+
+
 def traffic_from_profiles(name):
     p = os.path.join(ROOT, 'profiles', 'traffic.json')
     if os.path.exists(p):
@@ -164,12 +230,28 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--extra', default='d2q9_4096_e8,d3q19_bgk_256,d3q19_mrt_256_fp32s',
                     help='comma-separated extra workloads reported under "extra"')
+    ap.add_argument('--sweep', action='store_true',
+                    help='run the (G, R, L) sweep study (time to steady state, learned '
+                         'warm start vs plain) instead of the step benchmark')
+    ap.add_argument('--sweep-train', type=int, default=512)
+    ap.add_argument('--sweep-eval', type=int, default=128)
+    ap.add_argument('--sweep-Lmax', type=int, default=256)
+    ap.add_argument('--sweep-epochs', type=int, default=120)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path '
               '(see the sharding design); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get('RANK', '0'))
+    if args.sweep:
+        r = run_sweep_study(args.sweep_train, args.sweep_eval, args.sweep_Lmax,
+                            args.sweep_epochs, log=lambda m: print(m, file=sys.stderr, flush=True))
+        line = {'metric': 'sweep time-to-steady-state learned vs plain',
+                'value': round(r['speedup_warm_vs_plain'], 3), 'unit': 'x (plain / learned)',
+                'higher_is_better': True, 'n_gpus': 1, 'data': 'synthetic',
+                'dtype': 'f32+f64', 'config': {'workload': 'd2q9_sweep'}, 'study': r}
+        print(json.dumps(line), flush=True)
+        return
     name = args.workload
     w = WORKLOADS[name]
     r = run_gpu(name, args.steps, args.warmup)

@@ -41,8 +41,9 @@ struct idmx_sc_s {
   void* mrt_dev = nullptr;   // MRT tables in the working precision
   double last_ms = 0;
   const SimOps* ops = nullptr;
-  // fused launch shape (chosen at create time)
-  int tx = 0, ty = 0, H = 0;
+  // fused launch shape (chosen at create time, autotuner may override)
+  int tx = 0, ty = 0, H = 0, hist = 1;
+  int coll = 0;  // 0 BGK, 1 dense MRT, 2 diagonal MRT
 };
 
 // Per-(velocity set, E-stencil, precision, collision) entry points.
@@ -60,5 +61,7 @@ struct SimOps {
   int (*from_f64)(idmx_sc_s*, void* pop, const double* src);
 };
 
-const SimOps* get_ops_d2q9(int estencil, int precision, int collision);
-const SimOps* get_ops_d3q19(int estencil, int precision, int collision);
+const SimOps* get_ops_d2q9(int estencil, int precision, int coll);
+const SimOps* get_ops_d3q19_e4(int precision, int coll);
+const SimOps* get_ops_d3q19_e6(int precision, int coll);
+const SimOps* get_ops_d3q19_e8(int precision, int coll);

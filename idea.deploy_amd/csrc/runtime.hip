@@ -268,12 +268,14 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
   const int64_t V = (int64_t)d->dims[0] * d->dims[1] * (D == 3 ? d->dims[2] : 1);
   CHECK_ARG(V * Q < ((int64_t)1 << 40), "idmx_sc_create: lattice too large");
   if (d->collision == IDMX_MRT)
-    CHECK_ARG(d->mrt_A && d->mrt_beta && d->mrt_gamma, "idmx_sc_create: MRT tables missing");
+    CHECK_ARG(d->mrt_diag || (d->mrt_A && d->mrt_beta && d->mrt_gamma),
+              "idmx_sc_create: MRT tables missing");
 
   idmx_sc_s* s = new idmx_sc_s();
   s->ctx = c;
   s->desc = *d;
-  s->desc.mrt_A = s->desc.mrt_beta = s->desc.mrt_gamma = nullptr;
+  s->desc.mrt_A = s->desc.mrt_beta = s->desc.mrt_gamma = s->desc.mrt_diag = nullptr;
+  s->coll = d->collision == IDMX_BGK ? 0 : (d->mrt_diag ? 2 : 1);
   if (D == 2) s->desc.dims[2] = 1;
   if (d->precision == IDMX_PREC_F64) s->desc.rho0 = 0.0;
   s->D = D;
@@ -283,8 +285,10 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
   s->geo.L2 = D == 3 ? d->dims[2] : 1;
   s->geo.V = V;
   s->pop_elem = d->precision == IDMX_PREC_F64 ? 8 : 4;
-  s->ops = D == 2 ? get_ops_d2q9(d->estencil, d->precision, d->collision)
-                  : get_ops_d3q19(d->estencil, d->precision, d->collision);
+  if (D == 2) s->ops = get_ops_d2q9(d->estencil, d->precision, s->coll);
+  else if (d->estencil == IDMX_E4) s->ops = get_ops_d3q19_e4(d->precision, s->coll);
+  else if (d->estencil == IDMX_E6) s->ops = get_ops_d3q19_e6(d->precision, s->coll);
+  else s->ops = get_ops_d3q19_e8(d->precision, s->coll);
   if (!s->ops) {
     delete s;
     set_error("idmx_sc_create: unsupported combination");
@@ -310,11 +314,16 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
   }
   if (d->collision == IDMX_MRT) {
     const int NB = D == 2 ? 6 : 10;
-    const size_t cnt = (size_t)Q * Q + 2 * (size_t)NB * Q;
-    std::vector<double> hd(cnt);
-    memcpy(hd.data(), d->mrt_A, sizeof(double) * Q * Q);
-    memcpy(hd.data() + Q * Q, d->mrt_beta, sizeof(double) * NB * Q);
-    memcpy(hd.data() + Q * Q + NB * Q, d->mrt_gamma, sizeof(double) * NB * Q);
+    std::vector<double> hd;
+    if (s->coll == 2) {
+      hd.assign(d->mrt_diag, d->mrt_diag + Q);
+    } else {
+      hd.resize((size_t)Q * Q + 2 * (size_t)NB * Q);
+      memcpy(hd.data(), d->mrt_A, sizeof(double) * Q * Q);
+      memcpy(hd.data() + Q * Q, d->mrt_beta, sizeof(double) * NB * Q);
+      memcpy(hd.data() + Q * Q + NB * Q, d->mrt_gamma, sizeof(double) * NB * Q);
+    }
+    const size_t cnt = hd.size();
     hipError_t e = hipMalloc(&s->mrt_dev, cnt * s->pop_elem);
     if (e == hipSuccess) {
       if (s->pop_elem == 8) {
@@ -350,22 +359,28 @@ int idmx_sc_pop_bytes(idmx_sc_t s, size_t* bytes) {
   return IDMX_OK;
 }
 
-int idmx_sc_set_launch(idmx_sc_t s, int tx, int H) {
+int idmx_sc_set_launch(idmx_sc_t s, int tx, int ty, int H, int hist) {
   CHECK_ARG(s, "idmx_sc_set_launch: null simulation");
-  if (s->D == 2) CHECK_ARG(tx == 64 || tx == 256, "idmx_sc_set_launch: 2D tile width is 64 or 256");
-  else CHECK_ARG(tx == 32 || tx == 64, "idmx_sc_set_launch: 3D tile width is 32 (32x8) or 64 (64x4)");
+  if (s->D == 2)
+    CHECK_ARG((tx == 64 || tx == 128 || tx == 256) && ty == 1,
+              "idmx_sc_set_launch: 2D tiles are 64, 128 or 256 sites wide, ty = 1");
+  else
+    CHECK_ARG((tx == 32 && (ty == 8 || ty == 16)) || (tx == 64 && (ty == 4 || ty == 8)),
+              "idmx_sc_set_launch: 3D tiles are 32x8, 32x16, 64x4 or 64x8");
   CHECK_ARG(H >= 1, "idmx_sc_set_launch: H must be >= 1");
   s->tx = tx;
-  s->ty = s->D == 2 ? 1 : (tx == 64 ? 4 : 8);
+  s->ty = ty;
   s->H = H;
+  s->hist = hist ? 1 : 0;
   return IDMX_OK;
 }
 
-int idmx_sc_get_launch(idmx_sc_t s, int* tx, int* ty, int* H) {
-  CHECK_ARG(s && tx && ty && H, "idmx_sc_get_launch: invalid argument");
+int idmx_sc_get_launch(idmx_sc_t s, int* tx, int* ty, int* H, int* hist) {
+  CHECK_ARG(s && tx && ty && H && hist, "idmx_sc_get_launch: invalid argument");
   *tx = s->tx;
   *ty = s->ty;
   *H = s->H;
+  *hist = s->hist;
   return IDMX_OK;
 }
 

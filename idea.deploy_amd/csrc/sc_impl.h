@@ -29,7 +29,7 @@ struct Impl {
     return P;
   }
 
-  template <int TX, int TY, bool WF>
+  template <int TX, int TY, bool WF, bool HIST>
   static hipError_t launch_fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
                                  double* psi, double* u) {
     const Geom& g = s->geo;
@@ -40,21 +40,44 @@ struct Impl {
     const int nseg = (Lm + H - 1) / H;
     const int ntiles = ntx * nty * nseg;
     const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
-    hipLaunchKernelGGL((k_fused<S, ES, T, SHIFT, COLL, WF, TX, TY>), dim3(grid),
+    hipLaunchKernelGGL((k_fused<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST>), dim3(grid),
                        dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H,
                        ntx, nty, ntiles, n, psi, u);
     return hipGetLastError();
   }
 
+  // Tile shapes the autotuner chooses from (tx, ty, hist); the step that
+  // writes n/psi/u at a checkpoint uses one fixed shape.
   template <bool WF>
   static hipError_t fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
                           double* psi, double* u) {
-    if constexpr (S::D == 2) {
-      if (s->tx == 256) return launch_fused<256, 1, WF>(s, pop, swap, n, psi, u);
-      return launch_fused<64, 1, WF>(s, pop, swap, n, psi, u);
+    if constexpr (WF) {
+      if constexpr (S::D == 2) return launch_fused<64, 1, true, true>(s, pop, swap, n, psi, u);
+      else return launch_fused<32, 8, true, true>(s, pop, swap, n, psi, u);
     } else {
-      if (s->tx == 64) return launch_fused<64, 4, WF>(s, pop, swap, n, psi, u);
-      return launch_fused<32, 8, WF>(s, pop, swap, n, psi, u);
+      const bool h = s->hist != 0;
+      if constexpr (S::D == 2) {
+        switch (s->tx) {
+          case 64: return h ? launch_fused<64, 1, false, true>(s, pop, swap, n, psi, u)
+                            : launch_fused<64, 1, false, false>(s, pop, swap, n, psi, u);
+          case 128: return h ? launch_fused<128, 1, false, true>(s, pop, swap, n, psi, u)
+                             : launch_fused<128, 1, false, false>(s, pop, swap, n, psi, u);
+          default: return h ? launch_fused<256, 1, false, true>(s, pop, swap, n, psi, u)
+                            : launch_fused<256, 1, false, false>(s, pop, swap, n, psi, u);
+        }
+      } else {
+        const int code = s->tx * 100 + s->ty;
+        switch (code) {
+          case 6404: return h ? launch_fused<64, 4, false, true>(s, pop, swap, n, psi, u)
+                              : launch_fused<64, 4, false, false>(s, pop, swap, n, psi, u);
+          case 3216: return h ? launch_fused<32, 16, false, true>(s, pop, swap, n, psi, u)
+                              : launch_fused<32, 16, false, false>(s, pop, swap, n, psi, u);
+          case 6408: return h ? launch_fused<64, 8, false, true>(s, pop, swap, n, psi, u)
+                              : launch_fused<64, 8, false, false>(s, pop, swap, n, psi, u);
+          default: return h ? launch_fused<32, 8, false, true>(s, pop, swap, n, psi, u)
+                            : launch_fused<32, 8, false, false>(s, pop, swap, n, psi, u);
+        }
+      }
     }
   }
 
@@ -199,13 +222,17 @@ struct Impl {
 };
 
 // Table over precision and collision for one (S, ES).
+// coll: 0 BGK, 1 MRT with dense tables, 2 MRT with a diagonal S.
 template <class S, class ES>
-const SimOps* ops_for(int precision, int collision) {
-  if (precision == IDMX_PREC_F64)
-    return collision == IDMX_BGK ? Impl<S, ES, double, false, 0>::ops()
-                                 : Impl<S, ES, double, false, 1>::ops();
-  return collision == IDMX_BGK ? Impl<S, ES, float, true, 0>::ops()
-                               : Impl<S, ES, float, true, 1>::ops();
+const SimOps* ops_for(int precision, int coll) {
+  if (precision == IDMX_PREC_F64) {
+    if (coll == 0) return Impl<S, ES, double, false, 0>::ops();
+    if (coll == 1) return Impl<S, ES, double, false, 1>::ops();
+    return Impl<S, ES, double, false, 2>::ops();
+  }
+  if (coll == 0) return Impl<S, ES, float, true, 0>::ops();
+  if (coll == 1) return Impl<S, ES, float, true, 1>::ops();
+  return Impl<S, ES, float, true, 2>::ops();
 }
 
 }  // namespace idmx

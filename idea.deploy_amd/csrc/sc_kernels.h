@@ -60,6 +60,51 @@ __device__ inline int wrap(int v, int L) {
   return v >= L ? v - L : v;
 }
 
+// Coordinates of the equilibrium (c) and of the raw Guo term (d) on the
+// basis v_k = w * {1, xi_a, xi_a xi_b (a <= b)}: 2D {1, x, y, xx, yy, xy},
+// 3D {1, x, y, z, xx, yy, zz, xy, xz, yz}. With SHIFT the constant term uses
+// sum f~ (= n - rho0), i.e. the coordinates of f_eq - w rho0.
+template <class S, class T, bool SHIFT>
+__device__ inline void basis_coords(T dn, T n, T ux, T uy, T uz, T Fx, T Fy, T Fz, T uu,
+                                    T Fu, T* c, T* d) {
+  const T n0 = SHIFT ? dn : n;
+  c[0] = n0 - n * T(0.5 * kCM2) * uu;
+  d[0] = -T(kCM2) * Fu;
+  if (S::D == 2) {
+    c[1] = n * T(kCM2) * ux;  c[2] = n * T(kCM2) * uy;
+    c[3] = n * T(0.5 * kCM4) * ux * ux;
+    c[4] = n * T(0.5 * kCM4) * uy * uy;
+    c[5] = n * T(kCM4) * ux * uy;
+    d[1] = T(kCM2) * Fx;  d[2] = T(kCM2) * Fy;
+    d[3] = T(kCM4) * Fx * ux;
+    d[4] = T(kCM4) * Fy * uy;
+    d[5] = T(kCM4) * (Fx * uy + Fy * ux);
+  } else {
+    c[1] = n * T(kCM2) * ux;  c[2] = n * T(kCM2) * uy;  c[3] = n * T(kCM2) * uz;
+    c[4] = n * T(0.5 * kCM4) * ux * ux;
+    c[5] = n * T(0.5 * kCM4) * uy * uy;
+    c[6] = n * T(0.5 * kCM4) * uz * uz;
+    c[7] = n * T(kCM4) * ux * uy;
+    c[8] = n * T(kCM4) * ux * uz;
+    c[9] = n * T(kCM4) * uy * uz;
+    d[1] = T(kCM2) * Fx;  d[2] = T(kCM2) * Fy;  d[3] = T(kCM2) * Fz;
+    d[4] = T(kCM4) * Fx * ux;
+    d[5] = T(kCM4) * Fy * uy;
+    d[6] = T(kCM4) * Fz * uz;
+    d[7] = T(kCM4) * (Fx * uy + Fy * ux);
+    d[8] = T(kCM4) * (Fx * uz + Fz * ux);
+    d[9] = T(kCM4) * (Fy * uz + Fz * uy);
+  }
+}
+
+// True when moment k of some forcing basis vector is non-zero (Mp v_b).
+template <class S>
+__device__ constexpr bool mpv_row_nz(int k) {
+  for (int b = 0; b < NBasis<S::D>::value; b++)
+    if (S::MpV[b][k] != 0.0) return true;
+  return false;
+}
+
 // --------------------------------------------------------------------------
 // Collision of one site. f holds the (shifted when SHIFT) populations, dn is
 // sum_q f_q of those values, (gx, gy, gz) the isotropic gradient of psi and
@@ -107,39 +152,49 @@ __device__ inline void collide(const T (&f)[S::Q], T dn, T gx, T gy, T gz,
       const T fg = w * (T(kCM2) * xF + T(kCM4) * xF * xu - T(kCM2) * Fu);
       fo[q] = c_f * f[q] + om * feq + c_g * fg;
     }
+  } else if (COLL == 2) {
+    // Diagonal S in moment space with the bases as compile-time tables:
+    // f* = f - Mw^-1 [s o (Mw f - Mw f_eq)] + Mp^-1 [(1 - s/2) o (Mp F)],
+    // Mw f_eq = sum_k c_k (Mw v_k), Mp F = sum_k d_k (Mp v_k) (F the raw Guo
+    // term); P.mrtA holds the Q rates s.
+    constexpr int NB = NBasis<S::D>::value;
+    T c[NB], d[NB];
+    basis_coords<S, T, SHIFT>(dn, n, ux, uy, uz, Fx, Fy, Fz, uu, Fu, c, d);
+    const T* s = P.mrtA;
+    T md[Q], mg[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+      T m = 0;
+#pragma unroll
+      for (int q = 0; q < Q; q++)
+        if (S::Mw[k][q] != 0.0) m += T(S::Mw[k][q]) * f[q];
+#pragma unroll
+      for (int b = 0; b < NB; b++)
+        if (S::MwV[b][k] != 0.0) m -= T(S::MwV[b][k]) * c[b];
+      md[k] = s[k] * m;
+      T g = 0;
+#pragma unroll
+      for (int b = 0; b < NB; b++)
+        if (S::MpV[b][k] != 0.0) g += T(S::MpV[b][k]) * d[b];
+      mg[k] = (T(1) - T(0.5) * s[k]) * g;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      T a = f[q];
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (S::Mwi[q][k] != 0.0) a -= T(S::Mwi[q][k]) * md[k];
+#pragma unroll
+      for (int k = 0; k < Q; k++)
+        if (S::Mpi[q][k] != 0.0 && mpv_row_nz<S>(k)) a += T(S::Mpi[q][k]) * mg[k];
+      fo[q] = a;
+    }
   } else {
     // f* = A f + sum_k c_k beta_k + sum_k d_k gamma_k, with c_k / d_k the
     // coordinates of f_eq / raw Guo term on w∘{1, xi_a, xi_a xi_b}.
     constexpr int NB = NBasis<S::D>::value;
     T c[NB], d[NB];
-    const T n0 = SHIFT ? dn : n;
-    c[0] = n0 - n * T(0.5 * kCM2) * uu;
-    d[0] = -T(kCM2) * Fu;
-    if (S::D == 2) {
-      c[1] = n * T(kCM2) * ux;  c[2] = n * T(kCM2) * uy;
-      c[3] = n * T(0.5 * kCM4) * ux * ux;
-      c[4] = n * T(0.5 * kCM4) * uy * uy;
-      c[5] = n * T(kCM4) * ux * uy;
-      d[1] = T(kCM2) * Fx;  d[2] = T(kCM2) * Fy;
-      d[3] = T(kCM4) * Fx * ux;
-      d[4] = T(kCM4) * Fy * uy;
-      d[5] = T(kCM4) * (Fx * uy + Fy * ux);
-    } else {
-      c[1] = n * T(kCM2) * ux;  c[2] = n * T(kCM2) * uy;  c[3] = n * T(kCM2) * uz;
-      c[4] = n * T(0.5 * kCM4) * ux * ux;
-      c[5] = n * T(0.5 * kCM4) * uy * uy;
-      c[6] = n * T(0.5 * kCM4) * uz * uz;
-      c[7] = n * T(kCM4) * ux * uy;
-      c[8] = n * T(kCM4) * ux * uz;
-      c[9] = n * T(kCM4) * uy * uz;
-      d[1] = T(kCM2) * Fx;  d[2] = T(kCM2) * Fy;  d[3] = T(kCM2) * Fz;
-      d[4] = T(kCM4) * Fx * ux;
-      d[5] = T(kCM4) * Fy * uy;
-      d[6] = T(kCM4) * Fz * uz;
-      d[7] = T(kCM4) * (Fx * uy + Fy * ux);
-      d[8] = T(kCM4) * (Fx * uz + Fz * ux);
-      d[9] = T(kCM4) * (Fy * uz + Fz * uy);
-    }
+    basis_coords<S, T, SHIFT>(dn, n, ux, uy, uz, Fx, Fy, Fz, uu, Fu, c, d);
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       T a = 0;
@@ -188,7 +243,14 @@ __device__ inline void push_store(T* __restrict__ dst, const Geom& geo, int x,
 // Blocks are dispatched round-robin over the 8 XCDs; the tile order is
 // remapped so that each XCD walks a contiguous range of tiles (neighbouring
 // tiles share halo rows, which then hit the same L2).
-template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY>
+//
+// HIST selects where the tile's populations wait between their lead-plane
+// read and their collision: in registers (true; one HBM read per value, but
+// (R+1) Q values per thread held across the barrier), or nowhere (false; the
+// collision re-reads them, R iterations later, from L2 / the infinity
+// cache, which keeps the register count low enough for more waves per SIMD).
+template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY,
+          bool HIST = true>
 __global__ void __launch_bounds__(TX * TY)
 k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
         Phys<T> P, int H, int ntx, int nty, int ntiles,
@@ -258,8 +320,9 @@ k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
     hoff[k] = (int64_t)gx + (int64_t)geo.L0 * gy;
   }
 
-  T fh[R + 1][Q];  // own populations of the last R+1 lead planes
-  T dnh[R + 1];    // their sums
+  constexpr int HR = HIST ? R + 1 : 1;
+  T fh[HR][Q];  // own populations of the last R+1 lead planes (HIST)
+  T dnh[HR];    // their sums
 
   for (int j = 0; j < Hs + 2 * R; j++) {
     const int mj = ((m0 - R + j) % Lm + Lm) % Lm;
@@ -268,12 +331,10 @@ k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
     {
       T s = 0;
 #pragma unroll
-      for (int q = 0; q < Q; q++) {
-        fh[R][q] = pop[(size_t)q * geo.V + base + own_off];
-      }
+      for (int q = 0; q < Q; q++) fh[HR - 1][q] = pop[(size_t)q * geo.V + base + own_off];
 #pragma unroll
-      for (int q = 0; q < Q; q++) s += fh[R][q];
-      dnh[R] = s;
+      for (int q = 0; q < Q; q++) s += fh[HR - 1][q];
+      dnh[HR - 1] = s;
       lds[slot][D == 3 ? R + iy : 0][R + ix] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
     }
 #pragma unroll
@@ -288,6 +349,16 @@ k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
     __syncthreads();
     if (j >= 2 * R && own_valid) {
       // plane with lead index j - R
+      const int mc = m0 + j - 2 * R;
+      if (!HIST) {
+        const int64_t cbase = (int64_t)mc * PS;
+        T s = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) fh[0][q] = pop[(size_t)q * geo.V + cbase + own_off];
+#pragma unroll
+        for (int q = 0; q < Q; q++) s += fh[0][q];
+        dnh[0] = s;
+      }
       int sl[2 * R + 1];
 #pragma unroll
       for (int k = 0; k <= 2 * R; k++) sl[k] = (j - 2 * R + k) % NS;
@@ -307,7 +378,6 @@ k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
       const T lpsi = lds[sl[R]][cy][cx];
       T fo[Q], u[3];
       collide<S, T, SHIFT, COLL>(fh[0], dnh[0], gx, gy, gz, lpsi, P, fo, u);
-      const int mc = m0 + j - 2 * R;
       const int xc = x, yc = D == 3 ? y : mc, zc = D == 3 ? mc : 0;
       push_store<S, T>(pop_swap, geo, xc, yc, zc, fo);
       if (WF) {
@@ -320,11 +390,13 @@ k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
         if (D == 3) u_out[g + 2 * geo.V] = (double)u[2];
       }
     }
+    if (HIST) {
 #pragma unroll
-    for (int k = 0; k < R; k++) {
-      dnh[k] = dnh[k + 1];
+      for (int k = 0; k < HR - 1; k++) {
+        dnh[k] = dnh[k + 1];
 #pragma unroll
-      for (int q = 0; q < Q; q++) fh[k][q] = fh[k + 1][q];
+        for (int q = 0; q < Q; q++) fh[k][q] = fh[k + 1][q];
+      }
     }
   }
 }

@@ -477,7 +477,8 @@ class _ShanChenCommon(RootLB):
         S = np.asarray(S, dtype=np.float64)
         key = S.tobytes()
         if key not in self._mrt_engines:
-            self._mrt_engines[key] = self._make_engine(mrt=hermite.mrt_tables(self.xi_name, S))
+            self._mrt_engines[key] = self._make_engine(
+                mrt=hermite.mrt_kernel_tables(self.xi_name, S))
         return self._mrt_engines[key]
 
     def End(self):

@@ -44,10 +44,15 @@ class SCEngine:
         d.rho0 = self.rho0
         keep = []
         if mrt is not None:
-            for name, arr in zip(('mrt_A', 'mrt_beta', 'mrt_gamma'), mrt):
-                a = np.ascontiguousarray(arr, dtype=np.float64)
+            if isinstance(mrt, dict) and 'diag' in mrt:
+                a = np.ascontiguousarray(mrt['diag'], dtype=np.float64)
                 keep.append(a)
-                setattr(d, name, a.ctypes.data)
+                d.mrt_diag = a.ctypes.data
+            else:
+                for name, arr in zip(('mrt_A', 'mrt_beta', 'mrt_gamma'), mrt):
+                    a = np.ascontiguousarray(arr, dtype=np.float64)
+                    keep.append(a)
+                    setattr(d, name, a.ctypes.data)
         h = C.c_void_p()
         _lib.call('idmx_sc_create', tenet.handle, C.byref(d), C.byref(h))
         self.handle = h
@@ -106,13 +111,13 @@ class SCEngine:
         _lib.call('idmx_sc_last_run_ms', self.handle, C.byref(ms))
         return ms.value
 
-    def set_launch(self, tx, H):
-        _lib.call('idmx_sc_set_launch', self.handle, int(tx), int(H))
+    def set_launch(self, tx, ty, H, hist=1):
+        _lib.call('idmx_sc_set_launch', self.handle, int(tx), int(ty), int(H), int(hist))
 
     def get_launch(self):
-        tx, ty, H = C.c_int(), C.c_int(), C.c_int()
-        _lib.call('idmx_sc_get_launch', self.handle, C.byref(tx), C.byref(ty), C.byref(H))
-        return tx.value, ty.value, H.value
+        v = [C.c_int() for _ in range(4)]
+        _lib.call('idmx_sc_get_launch', self.handle, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
 
     # -- host views of a population buffer ---------------------------------
     def pop_weights(self):

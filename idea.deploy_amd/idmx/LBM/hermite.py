@@ -171,3 +171,13 @@ def mrt_tables(name, S):
     gamma = (G @ V.T).T
     return (np.ascontiguousarray(A), np.ascontiguousarray(beta),
             np.ascontiguousarray(gamma))
+
+
+def mrt_kernel_tables(name, S):
+    """What SCEngine passes to the kernel for relaxation matrix S: the Q
+    rates when S is diagonal (moment-space kernel with compiled-in bases),
+    else the dense (A, beta, gamma) tables."""
+    S = np.asarray(S, dtype=np.float64)
+    if np.count_nonzero(S - np.diag(np.diag(S))) == 0:
+        return {'diag': np.ascontiguousarray(np.diag(S))}
+    return mrt_tables(name, S)

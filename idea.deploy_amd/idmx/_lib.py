@@ -23,7 +23,8 @@ class ScDesc(C.Structure):
     _fields_ = [('xi', C.c_int), ('estencil', C.c_int), ('dims', C.c_int * 3),
                 ('psi_kind', C.c_int), ('SC_G', C.c_double), ('tau', C.c_double),
                 ('collision', C.c_int), ('precision', C.c_int), ('rho0', C.c_double),
-                ('mrt_A', C.c_void_p), ('mrt_beta', C.c_void_p), ('mrt_gamma', C.c_void_p)]
+                ('mrt_A', C.c_void_p), ('mrt_beta', C.c_void_p), ('mrt_gamma', C.c_void_p),
+                ('mrt_diag', C.c_void_p)]
 
 
 class BatchDomain(C.Structure):
@@ -58,9 +59,9 @@ _SIGS = [
     ('idmx_sc_create', C.c_int, [C.c_void_p, C.POINTER(ScDesc), c_void_pp]),
     ('idmx_sc_destroy', C.c_int, [C.c_void_p]),
     ('idmx_sc_pop_bytes', C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
-    ('idmx_sc_set_launch', C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ('idmx_sc_set_launch', C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     ('idmx_sc_get_launch', C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                     C.POINTER(C.c_int)]),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ('idmx_sc_init_pop', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ('idmx_sc_density_psi', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ('idmx_sc_force_collide_stream', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,

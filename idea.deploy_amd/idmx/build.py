@@ -24,7 +24,8 @@ FLAGS = ['-std=c++17', '-O3', '-fPIC', '--offload-arch=' + ARCH,
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-Wno-unused-result']
 
-SOURCES = ['runtime.hip', 'sc_d2q9.hip', 'sc_d3q19.hip', 'batch.hip', 'nn.hip']
+SOURCES = ['runtime.hip', 'sc_d2q9.hip', 'sc_d3q19_e4.hip', 'sc_d3q19_e6.hip',
+           'sc_d3q19_e8.hip', 'batch.hip', 'nn.hip']
 
 
 def _run(cmd):

@@ -118,11 +118,16 @@ class Batch:
 
 
 def _relax(batch, active, steps, tol, check_every, max_steps, grow_checks, hist, restart,
-           label):
+           label, plateau=False):
     """Runs the live domains of `batch` until each one's max |dn| < tol.
-    restart(i) is called for a domain whose residual keeps growing."""
+    A domain whose residual stays above twice its best value for
+    `grow_checks` consecutive checks, or turns non-finite, is handed to
+    restart(i) (the cold-start fallback). With plateau=True (the FP32 phase)
+    a domain also finishes once its residual is below 100 tol and has not
+    improved by 20 % for `grow_checks` checks: FP32 rounding sets a floor."""
     n = batch.n
-    last = np.full(n, np.inf)
+    best = np.full(n, np.inf)
+    stale = np.zeros(n, dtype=int)
     grows = np.zeros(n, dtype=int)
     fell_back = np.zeros(n, dtype=bool)
     gpu_ms = 0.0
@@ -132,27 +137,30 @@ def _relax(batch, active, steps, tol, check_every, max_steps, grow_checks, hist,
         gpu_ms += batch.run(check_every)
         steps[active] += check_every
         dn = batch.check()
-        bad = active & (~np.isfinite(dn) | (dn > last))
-        grows = np.where(bad, grows + 1, 0)
-        last = np.where(active, dn, last)
-        done = active & np.isfinite(dn) & (dn < tol)
+        fin = np.isfinite(dn)
+        grows = np.where(active & fin & (dn > 2 * best) & (dn > tol), grows + 1, 0)
+        stale = np.where(active & fin & (dn > 0.8 * best), stale + 1, 0)
+        best = np.where(active & fin, np.minimum(best, dn), best)
+        done = active & fin & (dn < tol)
+        if plateau:
+            done |= active & fin & (dn < 100 * tol) & (stale >= grow_checks)
         hist.append((label, int(steps.max()), int(active.sum()),
                      float(np.nanmax(np.where(active, dn, 0.0)))))
         changed = done.any()
         active = active & ~done
-        for i in np.nonzero(active & ((grows >= grow_checks) | ~np.isfinite(dn)))[0]:
+        for i in np.nonzero(active & ((grows >= grow_checks) | ~fin))[0]:
             restart(i)
             fell_back[i] = True
-            grows[i] = 0
-            last[i] = np.inf
-        if (steps >= max_steps).all() or (steps[active] >= max_steps).all():
+            grows[i] = stale[i] = 0
+            best[i] = np.inf
+        if not active.any() or (steps[active] >= max_steps).all():
             break
         if changed:
             batch.set_active(active)
     return active, gpu_ms, fell_back
 
 
-def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=2e-6,
+def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
               check_every=200, max_steps=400000, grow_checks=5, estencil='E4',
               launch=None, launch32=None, rho0=1.0):
     """Relax every point to steady state; returns SweepResult and the final
@@ -179,7 +187,7 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=2e-6,
 
         active = np.ones(n, dtype=bool)
         _, ms, fb = _relax(b32, active, steps, tol32, check_every, max_steps, grow_checks,
-                           hist, restart32, 'fp32')
+                           hist, restart32, 'fp32', plateau=True)
         gpu_ms += ms
         fb_all |= fb
         b64.copy_state_from(b32)
@@ -224,3 +232,43 @@ def sample_points(count, seed, Ls=tuple(range(64, 513, 32)), G_lit=(-5.5, -4.2),
         R = float(rng.uniform(Rfrac[0] * L, Rfrac[1] * L))
         out.append(SweepPoint(L=L, G=G, R=R))
     return out
+
+
+def plain_runs(tenet, points, tol=1e-10, check_every=200, max_steps=400000, starts=None):
+    """The unbatched baseline: one FP64 simulation per point, run the way a
+    notebook runs MainLoop with a convergence callback, i.e. checkpoints of
+    `check_every` steps and a device max |n - n_prev| check after each.
+    Returns (steps, converged, wall seconds, final densities)."""
+    import time as _t
+    from .IdpyCode import IdpyMemory
+    from .LBM._base import PopArray
+    from .LBM._engine import SCEngine
+    steps = np.zeros(len(points), dtype=np.int64)
+    conv = np.zeros(len(points), dtype=bool)
+    fields = []
+    t0 = _t.perf_counter()
+    for i, p in enumerate(points):
+        eng = SCEngine(tenet, 'D2Q9', 'D2E4', (p.L, p.L), 'exp', p.G, p.tau)
+        V = p.L * p.L
+        n0 = cold_start(p) if starts is None else starts[i]
+        n = IdpyMemory.OnDevice(n0, tenet)
+        nprev = IdpyMemory.OnDevice(n0, tenet)
+        psi = IdpyMemory.Zeros(V, np.float64, tenet)
+        u = IdpyMemory.Zeros(2 * V, np.float64, tenet)
+        a, b = PopArray(eng, tenet), PopArray(eng, tenet)
+        eng.init_pop(a, n, u)
+        while steps[i] < max_steps:
+            if eng.run(a, b, check_every):
+                a, b = b, a
+            steps[i] += check_every
+            eng.density_psi(n, psi, a)
+            d = IdpyMemory.MaxAbsDiff(n, nprev)
+            n, nprev = nprev, n
+            if d < tol:
+                conv[i] = True
+                break
+        fields.append(nprev.D2H().reshape(p.L, p.L))
+        for x in (n, nprev, psi, u, a, b):
+            x.Free()
+        eng.close()
+    return steps, conv, _t.perf_counter() - t0, fields

@@ -33,15 +33,16 @@ def _store(key, val):
 
 
 def sim_candidates(D, L_march):
-    tiles = (64, 256) if D == 2 else (32, 64)
-    Hs = [h for h in (8, 16, 32, 64, 128) if h <= L_march] or [L_march]
-    return [(t, h) for t in tiles for h in Hs]
+    tiles = ((64, 1), (128, 1), (256, 1)) if D == 2 else ((32, 8), (64, 4), (32, 16), (64, 8))
+    Hs = [h for h in (16, 32, 64) if h <= L_march] or [L_march]
+    return [(tx, ty, h, hist) for tx, ty in tiles for h in Hs for hist in (1, 0)]
 
 
 def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
-    """Times the fused step of an SCEngine for each (tile, H) candidate on
-    scratch populations initialised from n0 (default: uniform 1.0 plus
-    noise) and sets the fastest on the engine. Returns (best, timings)."""
+    """Times the fused step of an SCEngine for each (tile, H, hist)
+    candidate on scratch populations initialised from n0 (default: uniform
+    1.0 plus small noise) and sets the fastest on the engine. Returns (best,
+    timings in ms per step)."""
     from .IdpyCode import IdpyMemory
     from .LBM._base import PopArray
     key = 'sim|%s|%s|%s|%s|%s|%s' % (engine.tenet.GetDeviceName(), engine.xi_name,
@@ -61,17 +62,16 @@ def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
     u = IdpyMemory.Zeros(V * D, np.float64, t)
     a, b = PopArray(engine, t), PopArray(engine, t)
     engine.init_pop(a, n, u)
-    L_march = engine.dims[-1]
     timings = {}
     old = engine.get_launch()
-    for tile, H in sim_candidates(D, L_march):
-        engine.set_launch(tile, H)
+    for cand in sim_candidates(D, engine.dims[-1]):
+        engine.set_launch(*cand)
         engine.run(a, b, 2)
         engine.run(a, b, steps)
-        timings['%d,%d' % (tile, H)] = engine.last_run_ms() / steps
+        timings[','.join(map(str, cand))] = engine.last_run_ms() / steps
     best = min(timings, key=timings.get)
     best = tuple(int(v) for v in best.split(','))
-    engine.set_launch(*(best if apply else old[::2]))
+    engine.set_launch(*(best if apply else old))
     for x in (a, b, n, u):
         x.Free()
     _store(key, {'best': best, 'timings': timings})

@@ -86,6 +86,10 @@ typedef struct {
   const double* mrt_A;
   const double* mrt_beta;
   const double* mrt_gamma;
+  /* MRT with a diagonal S: its Q rates (then the three tables above may be
+   * NULL); the kernel works in moment space with the Hermite bases compiled
+   * in. */
+  const double* mrt_diag;
 } idmx_sc_desc;
 
 /* ---- library / context ---------------------------------------------- */
@@ -184,13 +188,14 @@ int idmx_sc_mass_momentum(idmx_sc_t sim, const void* pop, double* mass,
 int idmx_sc_pop_to_f64(idmx_sc_t sim, const void* pop, double* dst_dev);
 int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
 
-/* Launch shape of the fused step: tile width tx (2D: 64 or 256 sites per
- * row; 3D: 32 for 32x8 tiles or 64 for 64x4 tiles) and H, the number of
- * planes a block marches through. Chosen at creation; the autotuner
- * (idmx.tune) overrides it. No reference counterpart: the reference fixes
- * block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
-int idmx_sc_set_launch(idmx_sc_t sim, int tx, int H);
-int idmx_sc_get_launch(idmx_sc_t sim, int* tx, int* ty, int* H);
+/* Launch shape of the fused step: tile tx x ty (2D: tx = 64, 128 or 256
+ * sites of one row, ty = 1; 3D: 32x8, 32x16, 64x4, 64x8), H planes marched
+ * per block, hist = 1 to hold the tile's populations in registers between
+ * their read and their collision, 0 to re-read them. Chosen at creation;
+ * the autotuner (idmx.tune) overrides it. No reference counterpart: the
+ * reference fixes block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
+int idmx_sc_set_launch(idmx_sc_t sim, int tx, int ty, int H, int hist);
+int idmx_sc_get_launch(idmx_sc_t sim, int* tx, int* ty, int* H, int* hist);
 
 /* Milliseconds of the last idmx_sc_run, measured with HIP events recorded
  * on the simulation's stream around its kernels. */

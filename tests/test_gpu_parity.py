@@ -142,3 +142,56 @@ def test_fp32_shifted_close_to_fp64(tenet):
         out[prec] = eng.pop_to_host_soa(pop)
         eng.close()
     assert rel_err(out['fp32s'], out['fp64']) < 2e-5
+
+
+@pytest.mark.parametrize('xi,es,dims', [('D2Q9', 'D2E4', (37, 29)), ('D2Q9', 'D2E8', (300, 17)),
+                                        ('D3Q19', 'D3E4', (70, 19, 9)),
+                                        ('D3Q19', 'D3E8', (35, 17, 7))])
+def test_every_launch_variant_matches_oracle(tenet, xi, es, dims):
+    """Each tile shape, march length and register/re-read variant the
+    autotuner may pick gives the oracle's result."""
+    from idmx import tune
+    from idmx.LBM._engine import SCEngine
+    n0 = droplet_field(dims, seed=21)
+    f_ref, _, _, _ = _oracle(xi, es, dims, n0, 12)
+    eng = SCEngine(tenet, xi, es, dims, 'exp', G, 1.0)
+    for cand in tune.sim_candidates(len(dims), dims[-1]) + [
+            c[:2] + (3,) + c[3:] for c in tune.sim_candidates(len(dims), dims[-1])[:4]]:
+        print('variant', cand, flush=True)
+        eng.set_launch(*cand)
+        pop, swap, n, psi, u = _device_fields(tenet, eng, n0)
+        if eng.run(pop, swap, 12):
+            pop, swap = swap, pop
+        assert rel_err(eng.pop_to_host_soa(pop).ravel(), f_ref) <= TOL, cand
+        for x in (pop, swap, n, psi, u):
+            x.Free()
+    eng.close()
+
+
+@pytest.mark.parametrize('xi,es,dims,prec', [('D2Q9', 'D2E4', (24, 21), 'fp64'),
+                                             ('D3Q19', 'D3E4', (11, 10, 9), 'fp64'),
+                                             ('D3Q19', 'D3E8', (33, 17, 9), 'fp64'),
+                                             ('D3Q19', 'D3E4', (11, 10, 9), 'fp32s')])
+def test_mrt_diagonal_path_matches_oracle(tenet, xi, es, dims, prec):
+    """The moment-space MRT kernel (diagonal S, compiled-in bases) against the
+    oracle's dense matrices from its own sympy derivation."""
+    import sympy as sp
+    from idmx.LBM import hermite
+    from idmx.LBM._engine import SCEngine
+    from oracle import hermite_mrt
+    Q = 9 if xi == 'D2Q9' else 19
+    rng = np.random.default_rng(4)
+    s = rng.uniform(0.6, 1.6, size=Q)
+    tables = hermite.mrt_kernel_tables(xi, np.diag(s))
+    assert 'diag' in tables
+    n0 = droplet_field(dims, seed=12)
+    eng = SCEngine(tenet, xi, es, dims, 'exp', G, 1.0, mrt=tables, precision=prec, rho0=1.0)
+    pop, swap, n, psi, u = _device_fields(tenet, eng, n0)
+    if eng.run(pop, swap, 30):
+        pop, swap = swap, pop
+    Ssym = sp.diag(*[sp.Rational(v).limit_denominator(10 ** 15) for v in s])
+    mats = hermite_mrt.mrt_matrices(xi, Ssym)
+    f_ref, _, _, _ = _oracle(xi, es, dims, n0, 30, mrt=mats)
+    tol = 1e-11 if prec == 'fp64' else 5e-6
+    assert rel_err(eng.pop_to_host_soa(pop).ravel(), f_ref) <= tol
+    eng.close()
