@@ -63,7 +63,7 @@ def workload_desc(name, w):
     return d
 
 
-def run_gpu(name, steps, warmup, tune=True):
+def run_gpu(name, steps, warmup, tune=True, launch=None):
     from idmx import configs
     from idmx.IdpyCode import IdpyMemory, Tenet
     from idmx.LBM import hermite
@@ -77,7 +77,9 @@ def run_gpu(name, steps, warmup, tune=True):
     eng = SCEngine(tenet, w['xi'], w['es'], w['dims'], 'exp', configs.SC_G, 1.0,
                    precision=w['prec'], rho0=rho0, mrt=mrt)
     V, D = eng.V, eng.D
-    if tune:
+    if launch:
+        eng.set_launch(*launch)
+    elif tune:
         from idmx import tune as tuner
         tuner.tune_sim(eng, steps=5, use_cache=False)
     n = IdpyMemory.OnDevice(n0, tenet)
@@ -230,6 +232,10 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--extra', default='d2q9_4096_e8,d3q19_bgk_256,d3q19_mrt_256_fp32s',
                     help='comma-separated extra workloads reported under "extra"')
+    ap.add_argument('--no-tune', action='store_true',
+                    help='skip the launch-shape autotuner (use the creation default)')
+    ap.add_argument('--launch', default=None,
+                    help='fixed launch shape tx,ty,H,hist for the main workload')
     ap.add_argument('--sweep', action='store_true',
                     help='run the (G, R, L) sweep study (time to steady state, learned '
                          'warm start vs plain) instead of the step benchmark')
@@ -254,7 +260,8 @@ def main():
         return
     name = args.workload
     w = WORKLOADS[name]
-    r = run_gpu(name, args.steps, args.warmup)
+    launch = tuple(int(v) for v in args.launch.split(',')) if args.launch else None
+    r = run_gpu(name, args.steps, args.warmup, tune=not args.no_tune, launch=launch)
     achieved = r['B_alg'] * r['V'] / (r['kernel_ms'] / 1e3) / 1e9
     line = {
         'metric': 'MLUPS & %HBM roofline (D3Q19 SC)',
@@ -280,7 +287,8 @@ def main():
     }
     extra = {}
     for x in [e for e in args.extra.split(',') if e and e != name]:
-        rx = run_gpu(x, max(20, args.steps // 4), max(5, args.warmup // 4))
+        rx = run_gpu(x, max(20, args.steps // 4), max(5, args.warmup // 4),
+                     tune=not args.no_tune)
         ax = rx['B_alg'] * rx['V'] / (rx['kernel_ms'] / 1e3) / 1e9
         extra[x] = {'mlups': round(rx['mlups'], 1), 'hbm_frac': round(ax / PEAK_HBM_GBS, 4),
                     'achieved_GBs': round(ax, 1), 'kernel_ms': round(rx['kernel_ms'], 4),

@@ -151,7 +151,8 @@ k_batched_dn(const T* __restrict__ pop_all, const BatchDom* __restrict__ doms,
         for (int q = 0; q < 9; q++) s += (double)pop[(size_t)q * V + g];
         const double n = SHIFT ? s + d.rho0 : s;
         double* np = n_prev + d.site_off + g;
-        m = fmax(m, fabs(n - *np));
+        const double dd = fabs(n - *np);
+        m = dd == dd ? fmax(m, dd) : INFINITY;  // a NaN state reports +inf
         *np = n;
       }
     }

@@ -8,7 +8,9 @@
 // Tensors are NHWC. Activations and weights used by the MFMA are bf16
 // (stored as uint16 bit patterns); master weights, gradients and the loss
 // are FP32. Conv weights are [Cout][9*Cin] with k = tap*Cin + ci,
-// tap = 3*ky + kx. Cin must be a multiple of 4 (the MFMA K slice per lane).
+// tap = 3*ky + kx. When Cin is a multiple of 4 a lane's MFMA K slice is 4
+// channels of one tap (vector loads); other channel counts are handled
+// element by element.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -65,22 +67,38 @@ k_conv3x3_fwd(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
   const int co = ct * 16 + r;  // B column
   const bool cvalid = co < s.Cout;
   const int K = 9 * s.Cin;
+  const bool vec4 = (s.Cin & 3) == 0;
   v4f32 acc = {0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < K; k0 += 16) {
     const int k = k0 + 4 * kq;
     v4i16 a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
-    if (k < K) {
-      const int tap = k / s.Cin, ci = k % s.Cin;
-      const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-      if (pvalid && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W) {
-        const uint16_t* src = in + (((int64_t)n * s.H + yy) * s.W + xx) * s.Cin + ci;
+    if (vec4) {
+      // Cin % 4 == 0: the 4 k values of a lane are 4 channels of one tap
+      if (k < K) {
+        const int tap = k / s.Cin, ci = k % s.Cin;
+        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+        if (pvalid && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W) {
+          const uint16_t* src = in + (((int64_t)n * s.H + yy) * s.W + xx) * s.Cin + ci;
 #pragma unroll
-        for (int j = 0; j < 4; j++) a[j] = (short)src[j];
+          for (int j = 0; j < 4; j++) a[j] = (short)src[j];
+        }
+        if (cvalid) {
+          const uint16_t* wsrc = wt + (int64_t)co * K + k;
+#pragma unroll
+          for (int j = 0; j < 4; j++) b[j] = (short)wsrc[j];
+        }
       }
-      if (cvalid) {
-        const uint16_t* wsrc = wt + (int64_t)co * K + k;
+    } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) b[j] = (short)wsrc[j];
+      for (int j = 0; j < 4; j++) {
+        const int kk = k + j;
+        if (kk < K) {
+          const int tap = kk / s.Cin, ci = kk % s.Cin;
+          const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+          if (pvalid && yy >= 0 && yy < s.H && xx >= 0 && xx < s.W)
+            a[j] = (short)in[(((int64_t)n * s.H + yy) * s.W + xx) * s.Cin + ci];
+          if (cvalid) b[j] = (short)wt[(int64_t)co * K + kk];
+        }
       }
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
@@ -377,8 +395,8 @@ extern "C" {
 int idmx_nn_conv3x3_fwd(idmx_ctx_t c, const void* in, const void* wt, const float* bias,
                         void* out, float* out_f32, int N, int H, int W, int Cin, int Cout,
                         int relu) {
-  NCHECK(c && in && wt && out && N > 0 && H > 0 && W > 0 && Cin > 0 && Cout > 0 && Cin % 4 == 0,
-         "idmx_nn_conv3x3_fwd: invalid argument (Cin must be a multiple of 4)");
+  NCHECK(c && in && wt && out && N > 0 && H > 0 && W > 0 && Cin > 0 && Cout > 0,
+         "idmx_nn_conv3x3_fwd: invalid argument");
   ConvShape s{N, H, W, Cin, Cout};
   const int64_t waves = (int64_t)((N * H * W + 15) / 16) * ((Cout + 15) / 16);
   hipLaunchKernelGGL(k_conv3x3_fwd, dim3(g1(waves * 64)), dim3(256), 0, c->stream,
@@ -390,8 +408,8 @@ int idmx_nn_conv3x3_fwd(idmx_ctx_t c, const void* in, const void* wt, const floa
 
 int idmx_nn_conv3x3_bwd_data(idmx_ctx_t c, const void* dy, const float* w, void* wd_scratch,
                              void* dx, int N, int H, int W, int Cin, int Cout) {
-  NCHECK(c && dy && w && wd_scratch && dx && Cout % 4 == 0,
-         "idmx_nn_conv3x3_bwd_data: invalid argument (Cout must be a multiple of 4)");
+  NCHECK(c && dy && w && wd_scratch && dx && N > 0 && H > 0 && W > 0 && Cin > 0 && Cout > 0,
+         "idmx_nn_conv3x3_bwd_data: invalid argument");
   const int tot = Cin * 9 * Cout;
   hipLaunchKernelGGL(k_conv_flip_weights, dim3(g1(tot)), dim3(256), 0, c->stream, w,
                      (uint16_t*)wd_scratch, Cin, Cout);

@@ -27,9 +27,13 @@ k_reduce_partial(const double* __restrict__ a, const double* __restrict__ b,
   double v = OP == 0 ? 0.0 : -INFINITY;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
+    // max and max |a - b| report +inf for a NaN entry (fmax would skip it)
     if (OP == 0) v += a[i];
-    else if (OP == 1) v = fmax(v, a[i]);
-    else v = fmax(v, fabs(a[i] - b[i]));
+    else if (OP == 1) v = a[i] == a[i] ? fmax(v, a[i]) : INFINITY;
+    else {
+      const double d = fabs(a[i] - b[i]);
+      v = d == d ? fmax(v, d) : INFINITY;
+    }
   }
   v = block_reduce<OP == 0 ? 0 : 1>(v);
   if (threadIdx.x == 0) out[blockIdx.x] = v;

@@ -118,9 +118,10 @@ int idmx_memset(idmx_ctx_t ctx, void* dst, int value, size_t bytes);
 /* ---- reductions, FP64 (IdpyMemory.Sum/Max, IdpyMemory.py:865-889) ----- */
 /* sum of n float64 values */
 int idmx_sum_f64(idmx_ctx_t ctx, const double* x, int64_t n, double* out);
-/* max of n float64 values */
+/* max of n float64 values (+inf if any is NaN) */
 int idmx_max_f64(idmx_ctx_t ctx, const double* x, int64_t n, double* out);
-/* max_i |a_i - b_i| (the |delta rho|_max steady-state check) */
+/* max_i |a_i - b_i| (the |delta rho|_max steady-state check; +inf if any
+ * difference is NaN) */
 int idmx_max_abs_diff_f64(idmx_ctx_t ctx, const double* a, const double* b,
                           int64_t n, double* out);
 
@@ -248,8 +249,8 @@ int idmx_batch_last_run_ms(idmx_batch_t b, double* ms);
  * No reference counterpart (idpy has no learned component). Tensors are NHWC
  * on the device; activations and conv weights in bf16 (uint16 bit patterns),
  * master weights, gradients and losses in float32. A conv weight is
- * [Cout][9*Cin] with k = (3*ky + kx)*Cin + ci. Cin of a forward conv (and
- * Cout of its backward-data pass) must be a multiple of 4. */
+ * [Cout][9*Cin] with k = (3*ky + kx)*Cin + ci. Channel counts that are
+ * multiples of 4 take the vector-load path. */
 /* out = act(conv3x3(in, wt) + bias), zero padding 1; out_f32 (may be NULL)
  * receives the FP32 pre-activation; relu != 0 applies max(0, .). */
 int idmx_nn_conv3x3_fwd(idmx_ctx_t ctx, const void* in, const void* wt,
