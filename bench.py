@@ -213,6 +213,53 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     return out
 
 
+def calibrate(reps=10):
+    """Plain copy of 19 x 256^3 float64 values (2.55 GB read + 2.55 GB
+    written per launch): achievable HBM bandwidth and the reference launch
+    for the FETCH_SIZE / WRITE_SIZE counter scale."""
+    import ctypes as C
+    from idmx import _lib
+    from idmx.IdpyCode import IdpyMemory, Tenet
+    t = Tenet(0)
+    n = 19 * 256 ** 3
+    a = IdpyMemory.Zeros(n, np.float64, t)
+    b = IdpyMemory.Zeros(n, np.float64, t)
+    ms = C.c_double()
+    best = []
+    for _ in range(reps):
+        _lib.call('idmx_stream_copy_f64', t.handle, b.ptr, a.ptr, n, C.byref(ms))
+        best.append(ms.value)
+    a.Free()
+    b.Free()
+    t.End()
+    med = float(np.median(best))
+    return {'bytes_per_launch': 16 * n, 'ms_median': med,
+            'GBs': 16 * n / (med / 1e3) / 1e9}
+
+
+def tune_report(names):
+    """The autotuner's timing table (ms per step, % of roofline) for each
+    workload."""
+    from idmx import configs, tune
+    from idmx.IdpyCode import Tenet
+    from idmx.LBM import hermite
+    from idmx.LBM._engine import SCEngine
+    t = Tenet(0)
+    for name in names:
+        w = WORKLOADS[name]
+        mrt = hermite.mrt_kernel_tables(w['xi'], mrt_S(w['xi'])) if w['coll'] == 'mrt' else None
+        eng = SCEngine(t, w['xi'], w['es'], w['dims'], 'exp', configs.SC_G, 1.0,
+                       precision=w['prec'], rho0=1.0 if w['prec'] == 'fp32s' else 0.0, mrt=mrt)
+        best, tim = tune.tune_sim(eng, steps=5, use_cache=False)
+        B = 2 * eng.Q * (8 if w['prec'] == 'fp64' else 4)
+        print('%s best %s' % (name, best), flush=True)
+        for k, ms in sorted(tim.items(), key=lambda kv: kv[1]):
+            print('  %-16s %.4f ms  %.1f%%' % (k, ms, 100 * B * eng.V / (ms / 1e3) / 8e12),
+                  flush=True)
+        eng.close()
+    t.End()
+
+
 def traffic_from_profiles(name):
     p = os.path.join(ROOT, 'profiles', 'traffic.json')
     if os.path.exists(p):
@@ -236,6 +283,10 @@ def main():
                     help='skip the launch-shape autotuner (use the creation default)')
     ap.add_argument('--launch', default=None,
                     help='fixed launch shape tx,ty,H,hist for the main workload')
+    ap.add_argument('--calibrate', action='store_true',
+                    help='time the plain copy kernel (bandwidth / counter calibration)')
+    ap.add_argument('--tune-report', default=None,
+                    help='print the autotuner table for these comma-separated workloads')
     ap.add_argument('--sweep', action='store_true',
                     help='run the (G, R, L) sweep study (time to steady state, learned '
                          'warm start vs plain) instead of the step benchmark')
@@ -249,6 +300,12 @@ def main():
               '(see the sharding design); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get('RANK', '0'))
+    if args.calibrate:
+        print(json.dumps({'calibration': calibrate()}), flush=True)
+        return
+    if args.tune_report:
+        tune_report(args.tune_report.split(','))
+        return
     if args.sweep:
         r = run_sweep_study(args.sweep_train, args.sweep_eval, args.sweep_Lmax,
                             args.sweep_epochs, log=lambda m: print(m, file=sys.stderr, flush=True))

@@ -91,6 +91,14 @@ k_check_u(const double* __restrict__ u, double* __restrict__ old_u, int64_t V, i
   }
 }
 
+// Plain streaming copy, one element per thread: the calibration kernel for
+// achievable HBM bandwidth and for the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void __launch_bounds__(256)
+k_stream_copy(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 }  // namespace idmx
 
 using namespace idmx;
@@ -227,6 +235,22 @@ int idmx_max_f64(idmx_ctx_t c, const double* x, int64_t n, double* out) {
 int idmx_max_abs_diff_f64(idmx_ctx_t c, const double* a, const double* b, int64_t n,
                           double* out) {
   return reduce<2>(c, a, b, n, out);
+}
+
+int idmx_stream_copy_f64(idmx_ctx_t c, double* dst, const double* src, int64_t n,
+                         double* ms) {
+  CHECK_ARG(c && dst && src && n > 0 && ms, "idmx_stream_copy_f64: invalid argument");
+  const int64_t blocks = (n + 255) / 256;
+  CHECK_ARG(blocks < ((int64_t)1 << 31), "idmx_stream_copy_f64: too large");
+  IDMX_HIP(hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, c->stream, src, dst, n);
+  IDMX_HIP(hipGetLastError());
+  IDMX_HIP(hipEventRecord(c->ev1, c->stream));
+  IDMX_HIP(hipEventSynchronize(c->ev1));
+  float t = 0;
+  IDMX_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = t;
+  return IDMX_OK;
 }
 
 int idmx_check_u(idmx_ctx_t c, const double* u, double* old_u, int64_t V, int D,
@@ -375,7 +399,8 @@ int idmx_sc_set_launch(idmx_sc_t s, int tx, int ty, int H, int hist) {
   s->tx = tx;
   s->ty = ty;
   s->H = H;
-  s->hist = hist ? 1 : 0;
+  CHECK_ARG(hist >= 0 && hist <= 4, "idmx_sc_set_launch: variant is 0..4");
+  s->hist = hist;
   return IDMX_OK;
 }
 

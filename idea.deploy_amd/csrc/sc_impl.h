@@ -29,7 +29,7 @@ struct Impl {
     return P;
   }
 
-  template <int TX, int TY, bool WF, bool HIST>
+  template <int TX, int TY, bool WF, bool HIST, bool PF = false, bool O3 = false>
   static hipError_t launch_fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
                                  double* psi, double* u) {
     const Geom& g = s->geo;
@@ -40,14 +40,36 @@ struct Impl {
     const int nseg = (Lm + H - 1) / H;
     const int ntiles = ntx * nty * nseg;
     const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
-    hipLaunchKernelGGL((k_fused<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST>), dim3(grid),
-                       dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H,
-                       ntx, nty, ntiles, n, psi, u);
+    if constexpr (O3)
+      hipLaunchKernelGGL((k_fused_o3<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF>), dim3(grid),
+                         dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H, ntx, nty,
+                         ntiles, n, psi, u);
+    else
+      hipLaunchKernelGGL((k_fused<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF>), dim3(grid),
+                         dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H, ntx, nty,
+                         ntiles, n, psi, u);
     return hipGetLastError();
   }
 
-  // Tile shapes the autotuner chooses from (tx, ty, hist); the step that
-  // writes n/psi/u at a checkpoint uses one fixed shape.
+  // variant: 1 populations held in registers, 0 re-read, 2 re-read with
+  // prefetch of the next lead plane, 3 re-read with >= 3 waves/SIMD,
+  // 4 re-read with prefetch and >= 3 waves/SIMD
+  template <int TX, int TY>
+  static hipError_t variant(idmx_sc_s* s, const T* pop, T* swap) {
+    switch (s->hist) {
+      case 0: return launch_fused<TX, TY, false, false>(s, pop, swap, nullptr, nullptr, nullptr);
+      case 2: return launch_fused<TX, TY, false, false, true>(s, pop, swap, nullptr, nullptr,
+                                                              nullptr);
+      case 3: return launch_fused<TX, TY, false, false, false, true>(s, pop, swap, nullptr,
+                                                                     nullptr, nullptr);
+      case 4: return launch_fused<TX, TY, false, false, true, true>(s, pop, swap, nullptr,
+                                                                    nullptr, nullptr);
+      default: return launch_fused<TX, TY, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
+    }
+  }
+
+  // Tile shapes the autotuner chooses from; the step that writes n/psi/u at
+  // a checkpoint uses one fixed shape.
   template <bool WF>
   static hipError_t fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
                           double* psi, double* u) {
@@ -55,27 +77,18 @@ struct Impl {
       if constexpr (S::D == 2) return launch_fused<64, 1, true, true>(s, pop, swap, n, psi, u);
       else return launch_fused<32, 8, true, true>(s, pop, swap, n, psi, u);
     } else {
-      const bool h = s->hist != 0;
       if constexpr (S::D == 2) {
         switch (s->tx) {
-          case 64: return h ? launch_fused<64, 1, false, true>(s, pop, swap, n, psi, u)
-                            : launch_fused<64, 1, false, false>(s, pop, swap, n, psi, u);
-          case 128: return h ? launch_fused<128, 1, false, true>(s, pop, swap, n, psi, u)
-                             : launch_fused<128, 1, false, false>(s, pop, swap, n, psi, u);
-          default: return h ? launch_fused<256, 1, false, true>(s, pop, swap, n, psi, u)
-                            : launch_fused<256, 1, false, false>(s, pop, swap, n, psi, u);
+          case 64: return variant<64, 1>(s, pop, swap);
+          case 128: return variant<128, 1>(s, pop, swap);
+          default: return variant<256, 1>(s, pop, swap);
         }
       } else {
-        const int code = s->tx * 100 + s->ty;
-        switch (code) {
-          case 6404: return h ? launch_fused<64, 4, false, true>(s, pop, swap, n, psi, u)
-                              : launch_fused<64, 4, false, false>(s, pop, swap, n, psi, u);
-          case 3216: return h ? launch_fused<32, 16, false, true>(s, pop, swap, n, psi, u)
-                              : launch_fused<32, 16, false, false>(s, pop, swap, n, psi, u);
-          case 6408: return h ? launch_fused<64, 8, false, true>(s, pop, swap, n, psi, u)
-                              : launch_fused<64, 8, false, false>(s, pop, swap, n, psi, u);
-          default: return h ? launch_fused<32, 8, false, true>(s, pop, swap, n, psi, u)
-                            : launch_fused<32, 8, false, false>(s, pop, swap, n, psi, u);
+        switch (s->tx * 100 + s->ty) {
+          case 6404: return variant<64, 4>(s, pop, swap);
+          case 3216: return variant<32, 16>(s, pop, swap);
+          case 6408: return variant<64, 8>(s, pop, swap);
+          default: return variant<32, 8>(s, pop, swap);
         }
       }
     }

@@ -249,13 +249,18 @@ __device__ inline void push_store(T* __restrict__ dst, const Geom& geo, int x,
 // (R+1) Q values per thread held across the barrier), or nowhere (false; the
 // collision re-reads them, R iterations later, from L2 / the infinity
 // cache, which keeps the register count low enough for more waves per SIMD).
+//
+// PF (with HIST = false) issues the loads of the next lead plane right after
+// the barrier, before the collision, so their latency overlaps the collision
+// arithmetic and stores.
 template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY,
-          bool HIST = true>
-__global__ void __launch_bounds__(TX * TY)
-k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
-        Phys<T> P, int H, int ntx, int nty, int ntiles,
-        double* __restrict__ n_out, double* __restrict__ psi_out,
-        double* __restrict__ u_out) {
+          bool HIST, bool PF>
+__device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restrict__ pop_swap,
+                                           const Geom& geo, const Phys<T>& P, int H, int ntx,
+                                           int nty, int ntiles, double* __restrict__ n_out,
+                                           double* __restrict__ psi_out,
+                                           double* __restrict__ u_out) {
+  static_assert(!(PF && HIST), "prefetch is for the re-read variant");
   constexpr int D = S::D, Q = S::Q, R = ES::R;
   constexpr int NT = TX * TY;
   constexpr int PX = TX + 2 * R;
@@ -323,6 +328,12 @@ k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
   constexpr int HR = HIST ? R + 1 : 1;
   T fh[HR][Q];  // own populations of the last R+1 lead planes (HIST)
   T dnh[HR];    // their sums
+  T fn[PF ? Q : 1];  // prefetched own populations of the next lead plane
+  if (PF) {
+    const int64_t b0 = (int64_t)(((m0 - R) % Lm + Lm) % Lm) * PS;
+#pragma unroll
+    for (int q = 0; q < Q; q++) fn[q] = pop[(size_t)q * geo.V + b0 + own_off];
+  }
 
   for (int j = 0; j < Hs + 2 * R; j++) {
     const int mj = ((m0 - R + j) % Lm + Lm) % Lm;
@@ -330,8 +341,13 @@ k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
     const int slot = j % NS;
     {
       T s = 0;
+      if (PF) {
 #pragma unroll
-      for (int q = 0; q < Q; q++) fh[HR - 1][q] = pop[(size_t)q * geo.V + base + own_off];
+        for (int q = 0; q < Q; q++) fh[HR - 1][q] = fn[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++) fh[HR - 1][q] = pop[(size_t)q * geo.V + base + own_off];
+      }
 #pragma unroll
       for (int q = 0; q < Q; q++) s += fh[HR - 1][q];
       dnh[HR - 1] = s;
@@ -347,6 +363,11 @@ k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
       }
     }
     __syncthreads();
+    if (PF && j + 1 < Hs + 2 * R) {
+      const int64_t bn = (int64_t)(((m0 - R + j + 1) % Lm + Lm) % Lm) * PS;
+#pragma unroll
+      for (int q = 0; q < Q; q++) fn[q] = pop[(size_t)q * geo.V + bn + own_off];
+    }
     if (j >= 2 * R && own_valid) {
       // plane with lead index j - R
       const int mc = m0 + j - 2 * R;
@@ -399,6 +420,28 @@ k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo,
       }
     }
   }
+}
+
+template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY,
+          bool HIST = true, bool PF = false>
+__global__ void __launch_bounds__(TX * TY)
+k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<T> P, int H,
+        int ntx, int nty, int ntiles, double* __restrict__ n_out, double* __restrict__ psi_out,
+        double* __restrict__ u_out) {
+  fused_body<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF>(pop, pop_swap, geo, P, H, ntx, nty,
+                                                          ntiles, n_out, psi_out, u_out);
+}
+
+// Same kernel compiled for at least 3 waves per SIMD (the register budget
+// drops to 168 per lane).
+template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY,
+          bool HIST = true, bool PF = false>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(3)))
+k_fused_o3(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<T> P, int H,
+           int ntx, int nty, int ntiles, double* __restrict__ n_out,
+           double* __restrict__ psi_out, double* __restrict__ u_out) {
+  fused_body<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF>(pop, pop_swap, geo, P, H, ntx, nty,
+                                                          ntiles, n_out, psi_out, u_out);
 }
 
 // --------------------------------------------------------------------------

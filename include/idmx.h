@@ -125,6 +125,12 @@ int idmx_max_f64(idmx_ctx_t ctx, const double* x, int64_t n, double* out);
 int idmx_max_abs_diff_f64(idmx_ctx_t ctx, const double* a, const double* b,
                           int64_t n, double* out);
 
+/* dst[i] = src[i] for n float64 values with a plain one-element-per-thread
+ * kernel; *ms = its duration. Calibrates achievable HBM bandwidth and the
+ * FETCH_SIZE / WRITE_SIZE counters (2 n 8 bytes moved). */
+int idmx_stream_copy_f64(idmx_ctx_t ctx, double* dst, const double* src,
+                         int64_t n, double* ms);
+
 /* K_CheckUMeta followed by IdpyMemory.Sum/Max
  * (idpy/LBM/MultiPhaseLoopChecks.py:96-163): *sum_delta = sum_g sum_d
  * |u - old_u|, *max_u2 = max_g sum_d u^2; old_u is overwritten with u.
@@ -191,8 +197,10 @@ int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
 
 /* Launch shape of the fused step: tile tx x ty (2D: tx = 64, 128 or 256
  * sites of one row, ty = 1; 3D: 32x8, 32x16, 64x4, 64x8), H planes marched
- * per block, hist = 1 to hold the tile's populations in registers between
- * their read and their collision, 0 to re-read them. Chosen at creation;
+ * per block, and the variant `hist`: 1 holds the tile's populations in
+ * registers between their read and their collision, 0 re-reads them, 2
+ * re-reads them and prefetches the next plane before the collision, 3 and 4
+ * are 0 and 2 compiled for at least 3 waves per SIMD. Chosen at creation;
  * the autotuner (idmx.tune) overrides it. No reference counterpart: the
  * reference fixes block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
 int idmx_sc_set_launch(idmx_sc_t sim, int tx, int ty, int H, int hist);
