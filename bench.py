@@ -154,12 +154,13 @@ def cpu_baseline(name, budget_s=12.0):
 def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     """Time to steady state (|dn|_max < 1e-10 between checks 200 steps apart)
     of a (G, R, L) sweep of centred droplets: plain per-point FP64 runs from
-    the cold start vs batched FP32->FP64 runs from the cold start and from
-    the learned warm start. L uniform in {64, 96, ..., Lmax}, G uniform in
-    [-5.5, -4.2] (textbook normalisation, SC_G = G/3), R in [0.1L, 0.35L].
-    The warm-start net is trained first on equilibrated fields of other
-    points produced by the batched solver."""
-    from idmx import configs, sweep, warmstart
+    the cold start vs batched runs (FP64, and FP32 then FP64) from the cold
+    start and from the learned warm start. L uniform in {64, 96, ..., Lmax},
+    G uniform in [-5.5, -4.2] (textbook normalisation, SC_G = G/3), R in
+    [0.1L, 0.35L]. The warm-start net is trained first on equilibrated fields
+    of other points produced by the batched solver; the batch launch shapes
+    come from the autotuner (its time is reported, not counted)."""
+    from idmx import configs, sweep, tune, warmstart
     from idmx.IdpyCode import Tenet
     tenet = Tenet(0)
     Ls = tuple(range(64, Lmax + 1, 32))
@@ -167,7 +168,16 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     eval_pts = sweep.sample_points(n_eval, configs.SEED0 + 4, Ls=Ls)
     out = {'n_train': n_train, 'n_eval': n_eval, 'L': [Ls[0], Ls[-1]]}
     t0 = time.perf_counter()
-    res_tr, f_tr = sweep.run_sweep(tenet, train_pts, mode='fp32+fp64', tol=1e-8)
+    l64, tim64 = tune.tune_batch(tenet, eval_pts, 'fp64', use_cache=False)
+    l32, tim32 = tune.tune_batch(tenet, eval_pts, 'fp32s', use_cache=False)
+    out['tune_s'] = time.perf_counter() - t0
+    out['launch_fp64'] = {'best': l64, 'ms_per_step': tim64[','.join(map(str, l64))]}
+    out['launch_fp32s'] = {'best': l32, 'ms_per_step': tim32[','.join(map(str, l32))]}
+    log('batch launch fp64 %s %.4f ms/step, fp32s %s %.4f ms/step' % (
+        l64, out['launch_fp64']['ms_per_step'], l32, out['launch_fp32s']['ms_per_step']))
+    t0 = time.perf_counter()
+    lt, _ = tune.tune_batch(tenet, train_pts, 'fp64', use_cache=False)
+    res_tr, f_tr = sweep.run_sweep(tenet, train_pts, mode='fp64', tol=1e-8, launch=lt)
     out['train_data_s'] = time.perf_counter() - t0
     log('training fields: %.1f s, converged %d/%d' % (out['train_data_s'],
                                                        res_tr.converged.sum(), n_train))
@@ -178,36 +188,44 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     net, hist = warmstart.train(tenet, X, Y, epochs=epochs, batch=32, log=log)
     out['train_s'] = time.perf_counter() - t0
     out['train_loss'] = [float(hist[0]), float(hist[-1])]
-    # held-out image error of the net vs the cold start
-    Xe = np.stack([warmstart.input_image(p.G, p.R, p.L)[0] for p in eval_pts])
     # plain per-point FP64 runs from the cold start
     steps_p, conv_p, wall_p, f_p = sweep.plain_runs(tenet, eval_pts)
     out['plain_fp64'] = {'wall_s': wall_p, 'steps_mean': float(steps_p.mean()),
-                         'converged': int(conv_p.sum())}
+                         'steps_max': int(steps_p.max()), 'converged': int(conv_p.sum())}
     log('plain: %.1f s' % wall_p)
-    res_c, f_c = sweep.run_sweep(tenet, eval_pts, mode='fp32+fp64')
-    out['batched_cold'] = {'wall_s': res_c.wall_s, 'steps_mean': float(res_c.steps.mean()),
-                           'converged': int(res_c.converged.sum())}
-    log('batched cold: %.1f s' % res_c.wall_s)
     t0 = time.perf_counter()
     starts = warmstart.predict_fields(tenet, net, eval_pts)
     t_inf = time.perf_counter() - t0
-    res_w, f_w = sweep.run_sweep(tenet, eval_pts, start=starts, mode='fp32+fp64')
-    out['batched_warm'] = {'wall_s': res_w.wall_s + t_inf, 'inference_s': t_inf,
-                           'steps_mean': float(res_w.steps.mean()),
-                           'converged': int(res_w.converged.sum()),
-                           'fallbacks': int(res_w.fallback.sum())}
-    log('batched warm: %.1f s' % (res_w.wall_s + t_inf))
+    runs = {'batched_cold_fp64': (None, 'fp64'), 'batched_cold': (None, 'fp32+fp64'),
+            'batched_warm_fp64': (starts, 'fp64'), 'batched_warm': (starts, 'fp32+fp64')}
+    fields = {}
+    for key, (st, mode) in runs.items():
+        res, f = sweep.run_sweep(tenet, eval_pts, start=st, mode=mode, launch=l64,
+                                 launch32=l32)
+        fields[key] = f
+        wall = res.wall_s + (t_inf if st is not None else 0.0)
+        out[key] = {'wall_s': wall, 'gpu_ms': res.gpu_ms, 'steps_mean': float(res.steps.mean()),
+                    'steps_max': int(res.steps.max()), 'converged': int(res.converged.sum()),
+                    'fallbacks': int(res.fallback.sum()),
+                    'history': res.history[::max(1, len(res.history) // 12)] + res.history[-1:]}
+        if st is not None:
+            out[key]['inference_s'] = t_inf
+        log('%s: %.1f s, steps mean %.0f, converged %d' % (key, wall, res.steps.mean(),
+                                                           res.converged.sum()))
     # initial-field error of warm vs cold start against the converged fields
     err0 = lambda st, f: float(np.abs(st.ravel() - f.ravel()).max())
     out['start_err_cold'] = float(np.median([err0(sweep.cold_start(p), f)
                                              for p, f in zip(eval_pts, f_p)]))
     out['start_err_warm'] = float(np.median([err0(st, f) for st, f in zip(starts, f_p)]))
     dp = lambda f, p: sweep.laplace_dp(f, p.G)
-    rel = [abs(dp(fw, p) - dp(fp, p)) / abs(dp(fp, p)) for fw, fp, p in zip(f_w, f_p, eval_pts)]
-    out['laplace_rel_diff_max'] = float(np.max(rel))
-    out['speedup_warm_vs_plain'] = wall_p / out['batched_warm']['wall_s']
-    out['speedup_cold_batched_vs_plain'] = wall_p / res_c.wall_s
+    for key in runs:
+        rel = [abs(dp(fw, p) - dp(fp, p)) / abs(dp(fp, p))
+               for fw, fp, p in zip(fields[key], f_p, eval_pts)]
+        out[key]['laplace_rel_diff_max'] = float(np.max(rel))
+    for key in runs:
+        out['speedup_%s_vs_plain' % key] = wall_p / out[key]['wall_s']
+    out['speedup_warm_vs_plain'] = max(out['speedup_batched_warm_vs_plain'],
+                                       out['speedup_batched_warm_fp64_vs_plain'])
     net.free()
     tenet.End()
     return out

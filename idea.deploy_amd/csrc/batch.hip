@@ -14,7 +14,8 @@ using namespace idmx;
 struct idmx_batch_s {
   idmx_ctx_s* ctx = nullptr;
   int prec = 0, estencil = 4;
-  int tx = 128, H = 32;
+  int kind = 1;         // 0: marching items (tx x H rows), 1: tiles (tx x H sites)
+  int tx = 32, H = 8;
   int ndom = 0;
   std::vector<BatchDom> doms;
   std::vector<int> active;
@@ -45,8 +46,27 @@ hipError_t launch_step(idmx_batch_s* b, const void* in, void* out) {
   return hipGetLastError();
 }
 
+template <class ES, class T, bool SHIFT, int BX, int BY>
+hipError_t launch_tile(idmx_batch_s* b, const void* in, void* out) {
+  hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY>), dim3(b->grid), dim3(BX * BY), 0,
+                     b->ctx->stream, (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems);
+  return hipGetLastError();
+}
+
+// Tile shapes (tx x H) accepted by idmx_batch_set_launch for kind 1.
+bool tile_shape_ok(int tx, int H) {
+  return (tx == 64 && H == 4) || (tx == 32 && H == 8) || (tx == 16 && H == 16) ||
+         (tx == 32 && H == 4);
+}
+
 template <class ES, class T, bool SHIFT>
 hipError_t step_tx(idmx_batch_s* b, const void* in, void* out) {
+  if (b->kind == 1) {
+    if (b->tx == 64) return launch_tile<ES, T, SHIFT, 64, 4>(b, in, out);
+    if (b->tx == 16) return launch_tile<ES, T, SHIFT, 16, 16>(b, in, out);
+    if (b->H == 4) return launch_tile<ES, T, SHIFT, 32, 4>(b, in, out);
+    return launch_tile<ES, T, SHIFT, 32, 8>(b, in, out);
+  }
   switch (b->tx) {
     case 64: return launch_step<ES, T, SHIFT, 64>(b, in, out);
     case 256: return launch_step<ES, T, SHIFT, 256>(b, in, out);
@@ -68,21 +88,12 @@ hipError_t one_step(idmx_batch_s* b, const void* in, void* out) {
                                   : step_es<float, true>(b, in, out);
 }
 
-template <class T, bool SHIFT, int TX>
-hipError_t launch_dn(idmx_batch_s* b) {
-  hipLaunchKernelGGL((k_batched_dn<T, SHIFT, TX>), dim3(b->grid), dim3(TX), 0, b->ctx->stream,
-                     (const T*)b->pop[b->cur], b->d_doms, b->d_items, b->d_nitems, b->n_prev,
-                     b->d_dmax);
-  return hipGetLastError();
-}
-
 template <class T, bool SHIFT>
-hipError_t dn_tx(idmx_batch_s* b) {
-  switch (b->tx) {
-    case 64: return launch_dn<T, SHIFT, 64>(b);
-    case 256: return launch_dn<T, SHIFT, 256>(b);
-    default: return launch_dn<T, SHIFT, 128>(b);
-  }
+hipError_t launch_dn(idmx_batch_s* b) {
+  hipLaunchKernelGGL((k_batched_dn<T, SHIFT>), dim3(b->grid), dim3(256), 0, b->ctx->stream,
+                     (const T*)b->pop[b->cur], b->d_doms, b->d_items, b->d_nitems, b->tx,
+                     b->n_prev, b->d_dmax);
+  return hipGetLastError();
 }
 
 void drop_graph(idmx_batch_s* b) {
@@ -92,7 +103,8 @@ void drop_graph(idmx_batch_s* b) {
 }
 
 // Rebuilds the work-item list from the active flags: for every live domain,
-// x-chunks of TX sites and runs of H rows.
+// x-chunks of tx sites and runs of H rows (kind 0), or tx x H tiles (kind 1;
+// the last tile of a row or column may reach past the domain edge).
 int rebuild_items(idmx_batch_s* b) {
   std::vector<BatchItem> items;
   for (int d = 0; d < b->ndom; d++) {
@@ -208,11 +220,18 @@ int idmx_batch_destroy(idmx_batch_t b) {
   return IDMX_OK;
 }
 
-int idmx_batch_set_launch(idmx_batch_t b, int tx, int H, int grid) {
+int idmx_batch_set_launch(idmx_batch_t b, int kind, int tx, int H, int grid) {
   BCHECK(b, "idmx_batch_set_launch: null batch");
-  BCHECK(tx == 64 || tx == 128 || tx == 256, "idmx_batch_set_launch: tx is 64, 128 or 256");
-  BCHECK(H >= 1, "idmx_batch_set_launch: H >= 1");
+  BCHECK(kind == 0 || kind == 1, "idmx_batch_set_launch: kind is 0 (march) or 1 (tile)");
+  if (kind == 0) {
+    BCHECK(tx == 64 || tx == 128 || tx == 256, "idmx_batch_set_launch: march tx is 64, 128 or 256");
+    BCHECK(H >= 1, "idmx_batch_set_launch: H >= 1");
+  } else {
+    BCHECK(tile_shape_ok(tx, H),
+           "idmx_batch_set_launch: tile is 64x4, 32x8, 16x16 or 32x4");
+  }
   BCHECK(grid >= 0, "idmx_batch_set_launch: grid >= 0");
+  b->kind = kind;
   b->tx = tx;
   b->H = H;
   if (grid > 0) b->grid = grid;
@@ -220,8 +239,10 @@ int idmx_batch_set_launch(idmx_batch_t b, int tx, int H, int grid) {
   return rebuild_items(b);
 }
 
-int idmx_batch_get_launch(idmx_batch_t b, int* tx, int* H, int* grid, int* n_items) {
-  BCHECK(b && tx && H && grid && n_items, "idmx_batch_get_launch: invalid argument");
+int idmx_batch_get_launch(idmx_batch_t b, int* kind, int* tx, int* H, int* grid,
+                          int* n_items) {
+  BCHECK(b && kind && tx && H && grid && n_items, "idmx_batch_get_launch: invalid argument");
+  *kind = b->kind;
   *tx = b->tx;
   *H = b->H;
   *grid = b->grid;
@@ -327,7 +348,8 @@ int idmx_batch_check(idmx_batch_t b, double* dmax) {
   hipStream_t st = b->ctx->stream;
   IDMX_HIP(hipMemsetAsync(b->d_dmax, 0, sizeof(unsigned long long) * b->ndom, st));
   if (b->n_items > 0) {
-    hipError_t e = b->prec == IDMX_PREC_F64 ? dn_tx<double, false>(b) : dn_tx<float, true>(b);
+    hipError_t e =
+        b->prec == IDMX_PREC_F64 ? launch_dn<double, false>(b) : launch_dn<float, true>(b);
     if (e != hipSuccess) return hip_fail(e, "idmx_batch_check");
   }
   std::vector<unsigned long long> h(b->ndom);
@@ -342,13 +364,15 @@ int idmx_batch_check(idmx_batch_t b, double* dmax) {
   return IDMX_OK;
 }
 
-int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src) {
+int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src, int flags) {
   BCHECK(dst && src && dst->ndom == src->ndom, "idmx_batch_copy_state: batches differ");
+  BCHECK(flags == 0 || flags == IDMX_BATCH_COPY_REST, "idmx_batch_copy_state: unknown flags");
   for (int d = 0; d < dst->ndom; d++)
     BCHECK(dst->doms[d].L0 == src->doms[d].L0 && dst->doms[d].L1 == src->doms[d].L1,
            "idmx_batch_copy_state: domain sizes differ");
   IDMX_HIP(hipStreamSynchronize(src->ctx->stream));
   hipStream_t st = dst->ctx->stream;
+  const bool s32 = src->prec == IDMX_PREC_F32S, d32 = dst->prec == IDMX_PREC_F32S;
   for (int d = 0; d < dst->ndom; d++) {
     const BatchDom& S = src->doms[d];
     const BatchDom& D = dst->doms[d];
@@ -356,21 +380,65 @@ int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src) {
     const unsigned grid = (unsigned)std::min<int64_t>((V + 255) / 256, 4096);
     const void* in = (const char*)src->pop[src->cur] + src->elem * S.pop_off;
     void* out = (char*)dst->pop[dst->cur] + dst->elem * D.pop_off;
-    if (src->prec == IDMX_PREC_F32S && dst->prec == IDMX_PREC_F64)
+    double* nref = dst->n_prev + D.site_off;
+    if (flags == IDMX_BATCH_COPY_REST) {
+      if (s32 && !d32)
+        hipLaunchKernelGGL((k_batched_rest<float, true, double, false>), dim3(grid), dim3(256),
+                           0, st, (const float*)in, (double*)out, V, S.rho0, 0.0, nref);
+      else if (!s32 && d32)
+        hipLaunchKernelGGL((k_batched_rest<double, false, float, true>), dim3(grid), dim3(256),
+                           0, st, (const double*)in, (float*)out, V, 0.0, D.rho0, nref);
+      else if (!s32)
+        hipLaunchKernelGGL((k_batched_rest<double, false, double, false>), dim3(grid),
+                           dim3(256), 0, st, (const double*)in, (double*)out, V, 0.0, 0.0, nref);
+      else
+        hipLaunchKernelGGL((k_batched_rest<float, true, float, true>), dim3(grid), dim3(256), 0,
+                           st, (const float*)in, (float*)out, V, S.rho0, D.rho0, nref);
+      IDMX_HIP(hipGetLastError());
+      continue;
+    }
+    if (s32 && !d32)
       hipLaunchKernelGGL((k_batched_convert<float, true, double, false>), dim3(grid), dim3(256),
                          0, st, (const float*)in, (double*)out, V, S.rho0, 0.0);
-    else if (src->prec == IDMX_PREC_F64 && dst->prec == IDMX_PREC_F32S)
+    else if (!s32 && d32)
       hipLaunchKernelGGL((k_batched_convert<double, false, float, true>), dim3(grid), dim3(256),
                          0, st, (const double*)in, (float*)out, V, 0.0, D.rho0);
-    else if (src->prec == IDMX_PREC_F64)
+    else if (!s32)
       hipLaunchKernelGGL((k_batched_convert<double, false, double, false>), dim3(grid),
                          dim3(256), 0, st, (const double*)in, (double*)out, V, 0.0, 0.0);
     else
       hipLaunchKernelGGL((k_batched_convert<float, true, float, true>), dim3(grid), dim3(256), 0,
                          st, (const float*)in, (float*)out, V, S.rho0, D.rho0);
     IDMX_HIP(hipGetLastError());
-    IDMX_HIP(hipMemcpyAsync(dst->n_prev + D.site_off, src->n_prev + S.site_off,
-                            sizeof(double) * V, hipMemcpyDeviceToDevice, st));
+    IDMX_HIP(hipMemcpyAsync(nref, src->n_prev + S.site_off, sizeof(double) * V,
+                            hipMemcpyDeviceToDevice, st));
+  }
+  IDMX_HIP(hipStreamSynchronize(st));
+  return IDMX_OK;
+}
+
+int idmx_batch_symmetrize(idmx_batch_t b, int dom) {
+  BCHECK(b && dom >= -1 && dom < b->ndom, "idmx_batch_symmetrize: invalid argument");
+  hipStream_t st = b->ctx->stream;
+  for (int d = dom < 0 ? 0 : dom; d < (dom < 0 ? b->ndom : dom + 1); d++) {
+    if (dom < 0 && !b->active[d]) continue;
+    const BatchDom& D = b->doms[d];
+    const int64_t V = (int64_t)D.L0 * D.L1;
+    const unsigned grid = (unsigned)std::min<int64_t>((V + 255) / 256, 4096);
+    void* pop = (char*)b->pop[b->cur] + b->elem * D.pop_off;
+    double* nref = b->n_prev + D.site_off;
+    if (b->prec == IDMX_PREC_F64) {
+      hipLaunchKernelGGL((k_batched_density<double, false>), dim3(grid), dim3(256), 0, st,
+                         (const double*)pop, b->stage, V, 0.0);
+      hipLaunchKernelGGL((k_batched_symmetrize<double, false>), dim3(grid), dim3(256), 0, st,
+                         b->stage, (double*)pop, D.L0, D.L1, 0.0, nref);
+    } else {
+      hipLaunchKernelGGL((k_batched_density<float, true>), dim3(grid), dim3(256), 0, st,
+                         (const float*)pop, b->stage, V, D.rho0);
+      hipLaunchKernelGGL((k_batched_symmetrize<float, true>), dim3(grid), dim3(256), 0, st,
+                         b->stage, (float*)pop, D.L0, D.L1, D.rho0, nref);
+    }
+    IDMX_HIP(hipGetLastError());
   }
   IDMX_HIP(hipStreamSynchronize(st));
   return IDMX_OK;

@@ -4,13 +4,14 @@
 // The reference bakes V, L_i and STR_i into every kernel as #defines
 // (idpy/LBM/LBM.py:542-569, idpy/IdpyCode/IdpyCode.py:770-781), so each domain
 // size is a new compile and a new launch. Here the geometry is data: a table
-// of domain descriptors and a list of work items (domain, x-chunk of TX
-// sites, run of H rows). Persistent blocks stride over the item list, each
-// item is one march of the fused step (the same algorithm as the
-// single-domain kernel: LDS psi ring with halo, populations kept in
-// registers, push streaming). The number of live items is read from device
-// memory, so the host can drop converged domains between graph replays
-// without re-capturing the graph.
+// of domain descriptors and a list of work items (domain, x0, y0, rows).
+// Persistent blocks stride over the item list. Two item kinds: a march of
+// the fused step over H rows of a TX-wide chunk (the single-domain kernel's
+// algorithm: LDS psi ring with halo, populations kept in registers, push
+// streaming; large domains), or one BX x BY tile with its halo (small
+// domains, where a march would be a chain of dependent row loads). The
+// number of live items is read from device memory, so the host can drop
+// converged domains between graph replays without re-capturing the graph.
 #pragma once
 #include "sc_kernels.h"
 
@@ -127,44 +128,129 @@ k_batched(const T* __restrict__ pop_all, T* __restrict__ swap_all,
   }
 }
 
-// Per-domain max |n - n_prev| over the live items (FP64), n_prev <- n.
-// dmax holds the bit patterns of non-negative doubles, whose unsigned order
-// is their numeric order, so a 64-bit atomic max reduces them.
-template <class T, bool SHIFT, int TX>
-__global__ void __launch_bounds__(TX)
+// Non-negative modulo for coordinates any number of periods out of range
+// (a tile can be wider than a small domain).
+__device__ inline int pmod(int v, int L) {
+  const int r = v % L;
+  return r < 0 ? r + L : r;
+}
+
+// One BX x BY tile of one domain per work item, one site per thread. The
+// tile's psi and an R-wide ring of halo psi are stored in LDS (the halo
+// densities are summed from their populations, which neighbouring tiles also
+// read, so those loads mostly hit L2), one barrier, then every site of the
+// tile collides and push-streams. Every load of an item is independent of
+// the others, so a step costs about two memory latencies however few rows a
+// domain has; the marching kernel above pays H + 2R dependent row loads per
+// item and suits large domains instead.
+//
+// A thread whose site lies past the domain edge (a tile wider or taller than
+// what is left of the domain) still loads the periodic image of its
+// position: that is the halo its valid neighbours need. Only sites inside
+// the domain collide and store.
+template <class ES, class T, bool SHIFT, int BX, int BY>
+__global__ void __launch_bounds__(BX * BY)
+k_batched_tile(const T* __restrict__ pop_all, T* __restrict__ swap_all,
+               const BatchDom* __restrict__ doms, const BatchItem* __restrict__ items,
+               const int* __restrict__ n_items) {
+  using S = D2Q9;
+  constexpr int Q = S::Q, R = ES::R;
+  constexpr int WX = BX + 2 * R, WY = BY + 2 * R, NT = BX * BY;
+  constexpr int NH = WX * WY - NT;  // halo ring: 2R rows of WX, 2R columns of BY
+  __shared__ T lds[WY][WX];
+  const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
+  const int ni = *n_items;
+  for (int it = blockIdx.x; it < ni; it += gridDim.x) {
+    const BatchItem w = items[it];
+    const BatchDom d = doms[w.dom];
+    const Geom geo{d.L0, d.L1, 1, (int64_t)d.L0 * d.L1};
+    const Phys<T> P = dom_phys<T>(d);
+    const T* pop = pop_all + d.pop_off;
+    const int x = w.x0 + tx, y = w.m0 + ty;
+    T f[Q];
+    T s = 0;
+    {
+      const int64_t g = (int64_t)pmod(y, d.L1) * d.L0 + pmod(x, d.L0);
+#pragma unroll
+      for (int q = 0; q < Q; q++) f[q] = pop[(size_t)q * geo.V + g];
+#pragma unroll
+      for (int q = 0; q < Q; q++) s += f[q];
+      lds[ty + R][tx + R] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+    }
+    for (int h = threadIdx.x; h < NH; h += NT) {
+      int hx, hy;
+      if (h < 2 * R * WX) {  // R rows above the tile, then R rows below
+        const int r = h / WX;
+        hx = h - r * WX;
+        hy = r < R ? r : BY + r;
+      } else {  // R columns left and right of each tile row
+        const int k = h - 2 * R * WX;
+        const int r = k / (2 * R), c = k - r * (2 * R);
+        hy = R + r;
+        hx = c < R ? c : BX + c;
+      }
+      const int64_t g =
+          (int64_t)pmod(w.m0 - R + hy, d.L1) * d.L0 + pmod(w.x0 - R + hx, d.L0);
+      T sh = 0;
+#pragma unroll
+      for (int q = 0; q < Q; q++) sh += pop[(size_t)q * geo.V + g];
+      lds[hy][hx] = psi_of(P.psi_kind, SHIFT ? P.rho0 + sh : sh);
+    }
+    __syncthreads();
+    if (x < d.L0 && y < d.L1) {
+      T gx = 0, gy = 0;
+#pragma unroll
+      for (int e = 0; e < ES::QE; e++) {
+        const T v = lds[ty + R + ES::ey[e]][tx + R + ES::ex[e]];
+        const T wgt = T(ES::w[e]);
+        if (ES::ex[e] != 0) gx += T(ES::ex[e]) * wgt * v;
+        if (ES::ey[e] != 0) gy += T(ES::ey[e]) * wgt * v;
+      }
+      T fo[Q], u[3];
+      collide<S, T, SHIFT, 0>(f, s, gx, gy, T(0), lds[ty + R][tx + R], P, fo, u);
+      push_store<S, T>(swap_all + d.pop_off, geo, x, y, 0, fo);
+    }
+    __syncthreads();
+  }
+}
+
+// Per-domain max |n - n_prev| over the live items (FP64), n_prev <- n. An
+// item covers W columns (its x-chunk or tile width) and Hs rows; 256
+// threads stride over its sites. dmax holds the bit patterns of non-negative
+// doubles, whose unsigned order is their numeric order, so a 64-bit atomic
+// max reduces them.
+template <class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
 k_batched_dn(const T* __restrict__ pop_all, const BatchDom* __restrict__ doms,
-             const BatchItem* __restrict__ items, const int* __restrict__ n_items,
+             const BatchItem* __restrict__ items, const int* __restrict__ n_items, int W,
              double* __restrict__ n_prev, unsigned long long* __restrict__ dmax) {
   const int ni = *n_items;
+  __shared__ double part[4];
   for (int it = blockIdx.x; it < ni; it += gridDim.x) {
     const BatchItem w = items[it];
     const BatchDom d = doms[w.dom];
     const int64_t V = (int64_t)d.L0 * d.L1;
     const T* pop = pop_all + d.pop_off;
     double m = 0.0;
-    const int x = w.x0 + threadIdx.x;
-    if (x < d.L0) {
-      for (int r = 0; r < w.Hs; r++) {
-        const int64_t g = (int64_t)(w.m0 + r) * d.L0 + x;
-        double s = 0.0;
+    for (int k = threadIdx.x; k < W * w.Hs; k += 256) {
+      const int r = k / W, x = w.x0 + (k - r * W);
+      if (x >= d.L0 || w.m0 + r >= d.L1) continue;
+      const int64_t g = (int64_t)(w.m0 + r) * d.L0 + x;
+      double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 9; q++) s += (double)pop[(size_t)q * V + g];
-        const double n = SHIFT ? s + d.rho0 : s;
-        double* np = n_prev + d.site_off + g;
-        const double dd = fabs(n - *np);
-        m = dd == dd ? fmax(m, dd) : INFINITY;  // a NaN state reports +inf
-        *np = n;
-      }
+      for (int q = 0; q < 9; q++) s += (double)pop[(size_t)q * V + g];
+      const double n = SHIFT ? s + d.rho0 : s;
+      double* np = n_prev + d.site_off + g;
+      const double dd = fabs(n - *np);
+      m = dd == dd ? fmax(m, dd) : INFINITY;  // a NaN state reports +inf
+      *np = n;
     }
-    // block max (TX threads)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-    __shared__ double part[TX / 64];
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-      double r = part[0];
-      for (int k = 1; k < TX / 64; k++) r = fmax(r, part[k]);
+      const double r = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
       atomicMax(dmax + w.dom, (unsigned long long)__double_as_longlong(r));
     }
     __syncthreads();
@@ -214,6 +300,61 @@ k_batched_convert(const TI* __restrict__ src, TO* __restrict__ dst, int64_t V, d
       if (SO) f -= D2Q9::w[q] * rho_out;
       dst[(size_t)q * V + g] = (TO)f;
     }
+  }
+}
+
+// Rest equilibrium f = w n of the source batch's density (summed in FP64),
+// written in the destination's storage: drops the momentum the source
+// carried. FP32 rounding gives a droplet a small net momentum, which FP64
+// then conserves exactly, so the droplet would translate for ever.
+template <class TI, bool SI, class TO, bool SO>
+__global__ void __launch_bounds__(256)
+k_batched_rest(const TI* __restrict__ src, TO* __restrict__ dst, int64_t V, double rho_in,
+               double rho_out, double* __restrict__ n_ref) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) s += (double)src[(size_t)q * V + g];
+    const double n = SI ? s + rho_in : s;
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+      dst[(size_t)q * V + g] = (TO)(D2Q9::w[q] * (SO ? n - rho_out : n));
+    n_ref[g] = n;
+  }
+}
+
+// Projection of one domain's density onto the fields symmetric about its
+// centre c = (L0/2, L1/2) (integer division): the average over the mirror
+// images x -> 2 c0 - x, y -> 2 c1 - y (mod L) and, on square domains, the
+// transpose, i.e. the dihedral group of the square (8 images) or of the
+// rectangle (4). The populations become the rest equilibrium of the
+// projected density. n holds the current density (read only); the result
+// goes to pop and to the check reference n_ref. Averaging over a permutation
+// group of the sites keeps the total mass.
+template <class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_batched_symmetrize(const double* __restrict__ n, T* __restrict__ pop, int L0, int L1,
+                     double rho0, double* __restrict__ n_ref) {
+  const int64_t V = (int64_t)L0 * L1;
+  const int c0 = L0 / 2, c1 = L1 / 2;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int y = (int)(g / L0), x = (int)(g - (int64_t)y * L0);
+    const int xr = pmod(2 * c0 - x, L0), yr = pmod(2 * c1 - y, L1);
+    double s = n[(int64_t)y * L0 + x] + n[(int64_t)y * L0 + xr] + n[(int64_t)yr * L0 + x] +
+               n[(int64_t)yr * L0 + xr];
+    double m = 4.0;
+    if (L0 == L1) {
+      s += n[(int64_t)x * L0 + y] + n[(int64_t)x * L0 + yr] + n[(int64_t)xr * L0 + y] +
+           n[(int64_t)xr * L0 + yr];
+      m = 8.0;
+    }
+    const double v = s / m;
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+      pop[(size_t)q * V + g] = (T)(D2Q9::w[q] * (SHIFT ? v - rho0 : v));
+    n_ref[g] = v;
   }
 }
 

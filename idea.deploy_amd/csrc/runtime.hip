@@ -91,6 +91,34 @@ k_check_u(const double* __restrict__ u, double* __restrict__ old_u, int64_t V, i
   }
 }
 
+// K_CenterOfMass + IdpyMemory.Sum (idpy/LBM/MultiPhase.py:108-137): per
+// block partial sums of n and n * x_d, x_d the lattice coordinates of the
+// lexicographic index (PosFromIndex, MultiPhase.py:100-106; no periodic
+// unwrapping, as in the reference).
+__global__ void __launch_bounds__(256)
+k_com_partial(const double* __restrict__ n, int L0, int L1, int64_t V, double* __restrict__ out) {
+  double m = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const double v = n[g];
+    const int64_t r = g / L0;
+    m += v;
+    mx += v * (double)(g - r * L0);
+    my += v * (double)(r % L1);
+    mz += v * (double)(r / L1);
+  }
+  m = block_reduce<0>(m);
+  mx = block_reduce<0>(mx);
+  my = block_reduce<0>(my);
+  mz = block_reduce<0>(mz);
+  if (threadIdx.x == 0) {
+    out[4 * blockIdx.x] = m;
+    out[4 * blockIdx.x + 1] = mx;
+    out[4 * blockIdx.x + 2] = my;
+    out[4 * blockIdx.x + 3] = mz;
+  }
+}
+
 // Plain streaming copy, one element per thread: the calibration kernel for
 // achievable HBM bandwidth and for the FETCH_SIZE / WRITE_SIZE counters.
 __global__ void __launch_bounds__(256)
@@ -223,6 +251,28 @@ int idmx_d2d(idmx_ctx_t c, void* dst, const void* src, size_t bytes) {
 int idmx_memset(idmx_ctx_t c, void* dst, int value, size_t bytes) {
   CHECK_ARG(c && dst, "idmx_memset: invalid argument");
   IDMX_HIP(hipMemsetAsync(dst, value, bytes, c->stream));
+  return IDMX_OK;
+}
+
+int idmx_center_of_mass_f64(idmx_ctx_t c, const double* n, int L0, int L1, int L2,
+                            double* out) {
+  CHECK_ARG(c && n && out && L0 > 0 && L1 > 0 && L2 > 0,
+            "idmx_center_of_mass_f64: invalid argument");
+  const int64_t V = (int64_t)L0 * L1 * L2;
+  const unsigned grid = (unsigned)std::min<int64_t>((V + 255) / 256, 1024);
+  CHECK_ARG((size_t)grid * 4 <= c->partials_cap, "idmx_center_of_mass_f64: scratch too small");
+  hipLaunchKernelGGL(k_com_partial, dim3(grid), dim3(256), 0, c->stream, n, L0, L1, V,
+                     c->partials);
+  IDMX_HIP(hipGetLastError());
+  std::vector<double> h(4 * (size_t)grid);
+  IDMX_HIP(hipMemcpyAsync(h.data(), c->partials, sizeof(double) * h.size(),
+                          hipMemcpyDeviceToHost, c->stream));
+  IDMX_HIP(hipStreamSynchronize(c->stream));
+  double acc[4] = {0, 0, 0, 0};
+  for (unsigned b = 0; b < grid; b++)
+    for (int k = 0; k < 4; k++) acc[k] += h[4 * b + k];
+  out[0] = acc[0];
+  for (int k = 1; k < 4; k++) out[k] = acc[k] / acc[0];
   return IDMX_OK;
 }
 

@@ -73,18 +73,24 @@ def Const(shape, dtype=np.float64, const=0, tenet=None):
     return a
 
 
+def _need_f64(array, what):
+    if array.dtype != np.float64:
+        raise TypeError('%s: the device reductions take float64 arrays, got %s'
+                        % (what, array.dtype))
+
+
 def Sum(array):
     """Device sum in float64 (IdpyMemory.Sum, IdpyMemory.py:865-877)."""
-    if array.dtype != np.float64:
-        return array.D2H().sum()
+    _need_f64(array, 'Sum')
     out = C.c_double()
     _lib.call('idmx_sum_f64', array.tenet.handle, array.ptr, array.size, C.byref(out))
     return out.value
 
 
 def Max(array):
-    if array.dtype != np.float64:
-        return array.D2H().max()
+    """Device max in float64 (IdpyMemory.Max, IdpyMemory.py:879-889); +inf
+    if any entry is NaN."""
+    _need_f64(array, 'Max')
     out = C.c_double()
     _lib.call('idmx_max_f64', array.tenet.handle, array.ptr, array.size, C.byref(out))
     return out.value
@@ -92,6 +98,22 @@ def Max(array):
 
 def MaxAbsDiff(a, b):
     """max |a - b| on the device (float64 arrays of the same size)."""
+    _need_f64(a, 'MaxAbsDiff')
+    _need_f64(b, 'MaxAbsDiff')
+    if a.size != b.size:
+        raise ValueError('MaxAbsDiff: sizes differ')
     out = C.c_double()
     _lib.call('idmx_max_abs_diff_f64', a.tenet.handle, a.ptr, b.ptr, a.size, C.byref(out))
     return out.value
+
+
+def CenterOfMass(n, dims):
+    """(sum n, centre of mass) of a float64 device field of lattice size
+    dims (x fastest), reduced on the device (K_CenterOfMass + Sum,
+    idpy/LBM/MultiPhase.py:108-137)."""
+    _need_f64(n, 'CenterOfMass')
+    L = list(dims) + [1] * (3 - len(dims))
+    out = (C.c_double * 4)()
+    _lib.call('idmx_center_of_mass_f64', n.tenet.handle, n.ptr, int(L[0]), int(L[1]), int(L[2]),
+              out)
+    return out[0], tuple(out[1 + d] for d in range(len(dims)))

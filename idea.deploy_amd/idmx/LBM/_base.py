@@ -15,6 +15,7 @@ import sympy as sp
 
 from ..IdpyCode import GetParamsClean, GetTenet, HIP_T
 from ..IdpyCode import IdpyMemory
+from ..IdpyCode import IdpySims as IdpySimsIO
 from ..IdpyCode.IdpyMemory import IdpyArrayHIP
 from . import hermite
 from ._engine import SCEngine
@@ -80,7 +81,23 @@ class IdpySims:
         self.sims_idpy_memory = {}
         self.sims_dump_vars, self.sims_dump_idpy_memory = [], []
         self.sims_not_dump_vars, self.sims_not_dump_idpy_memory = [], []
+        self.sims_dump_vars_flag, self.sims_dump_idpy_memory_flag = True, True
         self.aux_idpy_memory, self.aux_vars = [], []
+
+    # -- snapshots (idpy/IdpyCode/IdpySims.py:73-176) ---------------------------
+    def DumpSnapshot(self, file_name=None, custom_types=None):
+        """<Class>/{vars, idpy_memory, custom_types} into file_name: HDF5 when
+        h5py is importable, otherwise the same keys in an npz container."""
+        IdpySimsIO.dump(self, file_name, custom_types)
+
+    def ReadSnapshotData(self, file_name=None, full_key=None):
+        return IdpySimsIO.read(self, file_name, full_key)
+
+    def CheckSnapshotData(self, file_name=None, full_key=None):
+        return IdpySimsIO.check(self, file_name, full_key)
+
+    def ReadSnapshotFull(self):
+        pass
 
     def CleanAuxilliary(self):
         for key in self.aux_vars:
@@ -159,6 +176,9 @@ def psi_kind_from(psi_sym, psi_code):
         % (psi_sym, psi_code))
 
 
+RootLB_Dump_def_name = 'RootLB_dump.hdf5'
+
+
 class RootLB(IdpySims):
     """Dimensions, strides, constants and field getters
     (idpy/LBM/LBM.py:464-667)."""
@@ -225,6 +245,12 @@ class RootLB(IdpySims):
         else:
             dims = np.flip(np.append(sizes, [D]))
         return u.reshape(dims)
+
+    def DumpPopSnapshot(self, file_name=None):
+        """DumpSnapshot with the simulation's custom_types
+        (idpy/LBM/LBM.py:605-607)."""
+        self.DumpSnapshot(file_name=RootLB_Dump_def_name if file_name is None else file_name,
+                          custom_types=self.custom_types)
 
     def GetPopulations(self):
         if 'set_ordering' not in self.params_dict:
@@ -429,10 +455,10 @@ class _ShanChenCommon(RootLB):
 
     # -- analysis -----------------------------------------------------------
     def ComputeCenterOfMass(self):
-        n = self.sims_idpy_memory['n'].D2H()
-        mass = n.sum()
-        xs = self._coords()
-        return mass, tuple(float((n * x).sum() / mass) for x in xs)
+        """(mass, centre of mass) of n, reduced on the device in FP64
+        (idpy/LBM/MultiPhase.py:108-137)."""
+        return IdpyMemory.CenterOfMass(self.sims_idpy_memory['n'],
+                                       self.sims_vars['dim_sizes'])
 
     def DeltaPLaplace(self, inside=None, outside=None):
         """Laplace pressure p(centre) - p(V-1) (idpy/LBM/MultiPhase.py:1690-1756)."""
@@ -455,6 +481,28 @@ class _ShanChenCommon(RootLB):
         self.sims_vars['n_in_n_out'] = nio
         p_in, p_out = self.PBulk(nio[0]), self.PBulk(nio[1])
         return p_in, p_out, p_in - p_out
+
+    def SurfaceTensionLaplace(self, n_g=None, n_l=None):
+        """Surface tension of a centred droplet from Laplace's law with the
+        equimolar radius, from FP64 device reductions of n (call
+        ComputeMomentsMeta first). n_l / n_g default to n at the centre and
+        at site V-1 (the DeltaPLaplace probes). The equimolar radius R
+        satisfies M = V n_g + (n_l - n_g) |B_R| with |B_R| = pi R^2 (2D) or
+        4 pi R^3 / 3 (3D); sigma = dP R (2D) or dP R / 2 (3D). Returns
+        (sigma, R, dP)."""
+        _p_in, _p_out, dP = self.DeltaPLaplace()
+        nio = self.sims_vars['n_in_n_out']
+        n_l = nio[0] if n_l is None else n_l
+        n_g = nio[1] if n_g is None else n_g
+        mass = self.sims_vars['mass']
+        vol = (mass - self.sims_vars['V'] * n_g) / (n_l - n_g)
+        if self.sims_vars['DIM'] == 2:
+            R = np.sqrt(vol / np.pi)
+            sigma = dP * R
+        else:
+            R = (3.0 * vol / (4.0 * np.pi)) ** (1.0 / 3.0)
+            sigma = 0.5 * dP * R
+        return float(sigma), float(R), float(dP)
 
     def PBulk(self, n):
         """p = n c2 + G e2 psi(n)^2 / 2 (idpy/LBM/MultiPhase.py:1759-1768)."""

@@ -50,6 +50,8 @@ _SIGS = [
     ('idmx_d2h', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ('idmx_d2d', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ('idmx_memset', C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t]),
+    ('idmx_center_of_mass_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                            c_double_p]),
     ('idmx_sum_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, c_double_p]),
     ('idmx_max_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, c_double_p]),
     ('idmx_max_abs_diff_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
@@ -81,14 +83,16 @@ _SIGS = [
     ('idmx_batch_create', C.c_int, [C.c_void_p, C.POINTER(BatchDomain), C.c_int, C.c_int,
                                     C.c_int, C.c_int, c_void_pp]),
     ('idmx_batch_destroy', C.c_int, [C.c_void_p]),
-    ('idmx_batch_set_launch', C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
-    ('idmx_batch_get_launch', C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p, c_int_p]),
+    ('idmx_batch_set_launch', C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ('idmx_batch_get_launch', C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p, c_int_p,
+                                          c_int_p]),
     ('idmx_batch_set_active', C.c_int, [C.c_void_p, c_int_p]),
     ('idmx_batch_set_density', C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ('idmx_batch_get_density', C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ('idmx_batch_run', C.c_int, [C.c_void_p, C.c_int]),
     ('idmx_batch_check', C.c_int, [C.c_void_p, C.c_void_p]),
-    ('idmx_batch_copy_state', C.c_int, [C.c_void_p, C.c_void_p]),
+    ('idmx_batch_copy_state', C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ('idmx_batch_symmetrize', C.c_int, [C.c_void_p, C.c_int]),
     ('idmx_batch_last_run_ms', C.c_int, [C.c_void_p, c_double_p]),
     ('idmx_nn_conv3x3_fwd', C.c_int, [C.c_void_p] * 6 + [C.c_int] * 6),
     ('idmx_nn_conv3x3_bwd_data', C.c_int, [C.c_void_p] * 5 + [C.c_int] * 5),

@@ -14,8 +14,10 @@ Modes
   FP32 + FP64: any start, FP32 shifted populations to tol32, then the state
                is converted to FP64 and relaxed to tol (FP64 fixes the final
                accuracy)
-A start whose residual grows for `grow_checks` consecutive checks (or turns
-non-finite) is reset to the cold start (the fallback of a bad warm start).
+A caller-supplied start whose residual stays above twice its best value for
+`grow_checks` consecutive checks (or turns non-finite) is reset to the cold
+start once (the fallback of a bad warm start). A cold start is never reset:
+its trajectory would repeat.
 """
 import ctypes as C
 import time
@@ -80,11 +82,13 @@ class Batch:
         except Exception:
             pass
 
-    def set_launch(self, tx, H, grid=0):
-        _lib.call('idmx_batch_set_launch', self.handle, int(tx), int(H), int(grid))
+    def set_launch(self, kind, tx, H, grid=0):
+        """kind 0: marching items of tx x H rows; kind 1: tx x H tiles."""
+        _lib.call('idmx_batch_set_launch', self.handle, int(kind), int(tx), int(H), int(grid))
 
     def get_launch(self):
-        v = [C.c_int() for _ in range(4)]
+        """(kind, tx, H, grid, live work items)."""
+        v = [C.c_int() for _ in range(5)]
         _lib.call('idmx_batch_get_launch', self.handle, *[C.byref(x) for x in v])
         return tuple(x.value for x in v)
 
@@ -113,29 +117,44 @@ class Batch:
         _lib.call('idmx_batch_check', self.handle, out.ctypes.data)
         return out
 
-    def copy_state_from(self, other):
-        _lib.call('idmx_batch_copy_state', self.handle, other.handle)
+    def symmetrize(self, dom=-1):
+        """Projects domain dom's density (-1: every live domain) onto the
+        fields symmetric about its centre; populations to rest equilibrium."""
+        _lib.call('idmx_batch_symmetrize', self.handle, int(dom))
+
+    def copy_state_from(self, other, rest=False):
+        """Populations of `other` (converted); rest=True writes the rest
+        equilibrium of its density instead (zero momentum)."""
+        _lib.call('idmx_batch_copy_state', self.handle, other.handle, 1 if rest else 0)
 
 
-def _relax(batch, active, steps, tol, check_every, max_steps, grow_checks, hist, restart,
-           label, plateau=False):
-    """Runs the live domains of `batch` until each one's max |dn| < tol.
-    A domain whose residual stays above twice its best value for
-    `grow_checks` consecutive checks, or turns non-finite, is handed to
-    restart(i) (the cold-start fallback). With plateau=True (the FP32 phase)
-    a domain also finishes once its residual is below 100 tol and has not
-    improved by 20 % for `grow_checks` checks: FP32 rounding sets a floor."""
+def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks, hist,
+           restart, label, plateau=False):
+    """Runs the live domains of `batch` until each one's max |dn| < tol, or
+    until every live domain has run max_steps steps in this call. A domain
+    on a caller-supplied start (warm[i]) whose residual stays above twice
+    its best value for `grow_checks` consecutive checks, or turns
+    non-finite, is handed to restart(i) (the cold-start fallback) once. A
+    cold domain that turns non-finite is dropped as failed. With
+    plateau=True (the FP32 phase) a domain also finishes once its residual
+    has not improved by 20 % for `grow_checks` checks: FP32 rounding sets a
+    floor below which it cannot go, and the FP64 phase takes over there.
+    Returns (still active, failed, fell back, GPU ms)."""
     n = batch.n
     best = np.full(n, np.inf)
     stale = np.zeros(n, dtype=int)
     grows = np.zeros(n, dtype=int)
     fell_back = np.zeros(n, dtype=bool)
+    failed = np.zeros(n, dtype=bool)
+    ran = np.zeros(n, dtype=np.int64)
     gpu_ms = 0.0
+    active = active.copy()
     batch.set_active(active)
     batch.check()  # reference densities
     while active.any():
         gpu_ms += batch.run(check_every)
         steps[active] += check_every
+        ran[active] += check_every
         dn = batch.check()
         fin = np.isfinite(dn)
         grows = np.where(active & fin & (dn > 2 * best) & (dn > tol), grows + 1, 0)
@@ -143,71 +162,98 @@ def _relax(batch, active, steps, tol, check_every, max_steps, grow_checks, hist,
         best = np.where(active & fin, np.minimum(best, dn), best)
         done = active & fin & (dn < tol)
         if plateau:
-            done |= active & fin & (dn < 100 * tol) & (stale >= grow_checks)
-        hist.append((label, int(steps.max()), int(active.sum()),
-                     float(np.nanmax(np.where(active, dn, 0.0)))))
-        changed = done.any()
-        active = active & ~done
-        for i in np.nonzero(active & ((grows >= grow_checks) | ~fin))[0]:
+            done |= active & fin & (stale >= grow_checks)
+        hist.append((label, int(steps[active].max()), int(active.sum()),
+                     float(np.max(np.where(active & fin, dn, 0.0)))))
+        bad = active & ~done & (~fin | (warm & (grows >= grow_checks)))
+        dead = bad & ~warm
+        failed |= dead
+        changed = (done | dead).any()
+        active = active & ~done & ~dead
+        for i in np.nonzero(bad & warm)[0]:
             restart(i)
+            warm[i] = False
             fell_back[i] = True
             grows[i] = stale[i] = 0
             best[i] = np.inf
-        if not active.any() or (steps[active] >= max_steps).all():
+        if not active.any() or (ran[active] >= max_steps).all():
             break
         if changed:
             batch.set_active(active)
-    return active, gpu_ms, fell_back
+    return active, failed, fell_back, gpu_ms
 
 
 def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
               check_every=200, max_steps=400000, grow_checks=5, estencil='E4',
-              launch=None, launch32=None, rho0=1.0):
+              launch=None, launch32=None, rho0=1.0, symmetric=True):
     """Relax every point to steady state; returns SweepResult and the final
-    FP64 densities. start: None (cold) or a list of initial densities.
-    mode: 'fp64' (FP64 only) or 'fp32+fp64'."""
+    FP64 densities. start: None (cold) or a list of initial densities (None
+    entries start cold). mode: 'fp64' (FP64 only) or 'fp32+fp64'.
+    max_steps bounds each phase. launch / launch32: (kind, tx, H, grid) of
+    the FP64 / FP32 batch (None keeps the library default).
+    symmetric: the points are centred droplets, whose steady state is
+    symmetric about the domain centre; warm starts and the FP32 -> FP64
+    hand-over are projected onto symmetric fields (a broken symmetry makes
+    the droplet drift towards a symmetric lattice position for ~1e5 steps)."""
     n = len(points)
     steps = np.zeros(n, dtype=np.int64)
     hist = []
     fb_all = np.zeros(n, dtype=bool)
+    fail_all = np.zeros(n, dtype=bool)
+    warm = np.array([start is not None and start[i] is not None for i in range(n)])
+    init = [cold_start(p) if not warm[i] else start[i] for i, p in enumerate(points)]
     t0 = time.perf_counter()
     b64 = Batch(tenet, points, 'fp64', estencil)
     if launch:
         b64.set_launch(*launch)
     gpu_ms = 0.0
-    if mode == 'fp32+fp64':
-        b32 = Batch(tenet, points, 'fp32s', estencil, rho0=rho0)
-        if launch32:
-            b32.set_launch(*launch32)
-        for i, p in enumerate(points):
-            b32.set_density(i, cold_start(p) if start is None else start[i])
-
-        def restart32(i):
-            b32.set_density(i, cold_start(points[i]))
-
-        active = np.ones(n, dtype=bool)
-        _, ms, fb = _relax(b32, active, steps, tol32, check_every, max_steps, grow_checks,
-                           hist, restart32, 'fp32', plateau=True)
-        gpu_ms += ms
-        fb_all |= fb
-        b64.copy_state_from(b32)
-        b32.close()
-    else:
-        for i, p in enumerate(points):
-            b64.set_density(i, cold_start(p) if start is None else start[i])
 
     def restart64(i):
         b64.set_density(i, cold_start(points[i]))
 
-    active = np.ones(n, dtype=bool)
-    rem, ms, fb = _relax(b64, active, steps, tol, check_every, max_steps, grow_checks, hist,
-                         restart64, 'fp64')
+    if mode == 'fp32+fp64':
+        b32 = Batch(tenet, points, 'fp32s', estencil, rho0=rho0)
+        if launch32:
+            b32.set_launch(*launch32)
+        for i in range(n):
+            b32.set_density(i, init[i])
+            if symmetric and warm[i]:
+                b32.symmetrize(i)
+
+        def restart32(i):
+            b32.set_density(i, cold_start(points[i]))
+
+        _, failed, fb, ms = _relax(b32, np.ones(n, dtype=bool), warm, steps, tol32, check_every,
+                                   max_steps, grow_checks, hist, restart32, 'fp32',
+                                   plateau=True)
+        gpu_ms += ms
+        fb_all |= fb
+        # FP32 rounding leaves each domain a small net momentum that FP64
+        # would conserve (a droplet that never stops): hand over the density
+        # only, at rest
+        b64.copy_state_from(b32, rest=True)
+        b32.close()
+        if symmetric:
+            b64.symmetrize(-1)
+        for i in np.nonzero(failed)[0]:   # FP32 blew up: FP64 from the cold start
+            restart64(i)
+            fb_all[i] = True
+    elif mode == 'fp64':
+        for i in range(n):
+            b64.set_density(i, init[i])
+            if symmetric and warm[i]:
+                b64.symmetrize(i)
+    else:
+        raise ValueError('mode is fp64 or fp32+fp64')
+    rem, failed, fb, ms = _relax(b64, np.ones(n, dtype=bool), warm, steps, tol, check_every,
+                                 max_steps, grow_checks, hist, restart64, 'fp64')
     gpu_ms += ms
     fb_all |= fb
+    fail_all |= failed
     fields = [b64.get_density(i) for i in range(n)]
     b64.close()
     wall = time.perf_counter() - t0
-    return SweepResult(steps=steps, converged=~rem, fallback=fb_all, wall_s=wall,
+    return SweepResult(steps=steps, converged=~rem & ~fail_all, fallback=fb_all, wall_s=wall,
                        gpu_ms=gpu_ms, history=hist), fields
 
 

@@ -78,13 +78,21 @@ def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
     return best, timings
 
 
+def batch_candidates(cus):
+    """(kind, tx, H, grid): marching items and tiles, three persistent grid
+    sizes each."""
+    shapes = [(0, tx, H) for tx in (64, 128, 256) for H in (8, 16, 32)]
+    shapes += [(1, 64, 4), (1, 32, 8), (1, 16, 16), (1, 32, 4)]
+    return [s + (g,) for s in shapes for g in (cus * 4, cus * 8, cus * 16)]
+
+
 def tune_batch(tenet, points, precision, estencil='E4', steps=20, use_cache=True):
-    """Times k_batched for (TX, H, grid) candidates on a scratch batch of the
-    same domains (cold-start droplets) and returns the fastest
-    (tx, H, grid) plus all timings (ms per step)."""
+    """Times the batched step for every (kind, tx, H, grid) candidate on a
+    scratch batch of the same domains (cold-start droplets) and returns the
+    fastest plus all timings (ms per step)."""
     from .sweep import Batch, cold_start
     sig = '%d:%d:%d' % (len(points), sum(p.L * p.L for p in points), max(p.L for p in points))
-    key = 'batch|%s|%s|%s|%s' % (tenet.GetDeviceName(), precision, estencil, sig)
+    key = 'batch2|%s|%s|%s|%s' % (tenet.GetDeviceName(), precision, estencil, sig)
     if use_cache:
         c = _load().get(key)
         if c:
@@ -92,15 +100,12 @@ def tune_batch(tenet, points, precision, estencil='E4', steps=20, use_cache=True
     b = Batch(tenet, points, precision, estencil)
     for i, p in enumerate(points):
         b.set_density(i, cold_start(p))
-    cus = b.get_launch()[2] // 8
+    cus = b.get_launch()[3] // 8
     timings = {}
-    for tx in (64, 128, 256):
-        for H in (16, 32, 64):
-            for g in (cus * 4, cus * 8, cus * 16):
-                b.set_launch(tx, H, g)
-                b.run(2)
-                ms = b.run(steps) / steps
-                timings['%d,%d,%d' % (tx, H, g)] = ms
+    for cand in batch_candidates(cus):
+        b.set_launch(*cand)
+        b.run(2)
+        timings[','.join(map(str, cand))] = b.run(steps) / steps
     b.close()
     best = min(timings, key=timings.get)
     best = tuple(int(v) for v in best.split(','))

@@ -125,6 +125,13 @@ int idmx_max_f64(idmx_ctx_t ctx, const double* x, int64_t n, double* out);
 int idmx_max_abs_diff_f64(idmx_ctx_t ctx, const double* a, const double* b,
                           int64_t n, double* out);
 
+/* K_CenterOfMass + IdpyMemory.Sum (idpy/LBM/MultiPhase.py:108-137) over a
+ * float64 density field of L0 x L1 x L2 sites (x fastest; L2 = 1 in 2D):
+ * out[0] = sum n, out[1 + d] = sum n x_d / sum n, coordinates not
+ * unwrapped across the periodic boundary. */
+int idmx_center_of_mass_f64(idmx_ctx_t ctx, const double* n, int L0, int L1,
+                            int L2, double* out);
+
 /* dst[i] = src[i] for n float64 values with a plain one-element-per-thread
  * kernel; *ms = its duration. Calibrates achievable HBM bandwidth and the
  * FETCH_SIZE / WRITE_SIZE counters (2 n 8 bytes moved). */
@@ -229,11 +236,14 @@ int idmx_batch_create(idmx_ctx_t ctx, const idmx_batch_domain* doms, int ndom,
                       int precision, int estencil, int psi_kind,
                       idmx_batch_t* out);
 int idmx_batch_destroy(idmx_batch_t b);
-/* Work decomposition: x-chunks of tx sites (64, 128, 256), runs of H rows,
- * grid persistent blocks (0 keeps the current grid). Picked by the
- * autotuner (idmx.tune). */
-int idmx_batch_set_launch(idmx_batch_t b, int tx, int H, int grid);
-int idmx_batch_get_launch(idmx_batch_t b, int* tx, int* H, int* grid,
+/* Work decomposition. kind 0 (march): items are x-chunks of tx sites
+ * (64, 128, 256) by runs of H rows, each marched row by row with the psi
+ * ring in LDS (large domains). kind 1 (tile, the default 32 x 8): items are
+ * tx x H tiles (64x4, 32x8, 16x16, 32x4), one site per thread, halo psi in
+ * LDS (small domains: no dependent row loads). grid = persistent blocks (0
+ * keeps the current grid). Picked by the autotuner (idmx.tune). */
+int idmx_batch_set_launch(idmx_batch_t b, int kind, int tx, int H, int grid);
+int idmx_batch_get_launch(idmx_batch_t b, int* kind, int* tx, int* H, int* grid,
                           int* n_items);
 /* active[d] != 0 keeps domain d in the work list. */
 int idmx_batch_set_active(idmx_batch_t b, const int* active);
@@ -249,8 +259,21 @@ int idmx_batch_run(idmx_batch_t b, int nsteps);
  * inactive domains. */
 int idmx_batch_check(idmx_batch_t b, double* dmax);
 /* Copies the state of every domain from src to dst (same domains; FP32
- * shifted <-> FP64 conversion included). */
-int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src);
+ * shifted <-> FP64 conversion included). flags = IDMX_BATCH_COPY_REST
+ * writes the rest equilibrium f = w n(src) instead (zero momentum: the
+ * FP32 -> FP64 hand-over of a sweep, where FP32 rounding has left a small
+ * net momentum that FP64 would conserve). */
+#define IDMX_BATCH_COPY_REST 1
+int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src, int flags);
+/* Projects the density of domain dom (-1: every live domain) onto the
+ * fields symmetric about its centre (L0/2, L1/2): the average over the
+ * mirror images in x and y and, for square domains, the transpose. The
+ * populations become the rest equilibrium of the projected density and it
+ * becomes the check reference; the total mass is unchanged. For centred
+ * droplets, whose steady state has this symmetry: FP32 rounding or an
+ * approximate start breaks it, and an off-centre droplet drifts towards a
+ * symmetric lattice position over ~1e5 steps. */
+int idmx_batch_symmetrize(idmx_batch_t b, int dom);
 int idmx_batch_last_run_ms(idmx_batch_t b, double* ms);
 
 /* ---- warm-start network kernels ---------------------------------------

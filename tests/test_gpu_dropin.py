@@ -85,3 +85,55 @@ def test_meta_mainloop_does_not_write_u_and_checks_run():
     p_in, p_out, dp = sim.DeltaPLaplace()
     assert dp > 0
     sim.End()
+
+
+def test_center_of_mass_and_laplace_surface_tension():
+    """Device centre of mass equals the numpy one on the same field; the
+    Laplace surface tension sigma = dP R_eq is the same for two droplet
+    radii once both have relaxed (Laplace's law), and dP > 0."""
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    n = sp.Symbol('n')
+    out = []
+    for R in (12.0, 20.0):
+        sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=(96, 96), xi_stencil=XIStencils['D2Q9'],
+                                 f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln_0)',
+                                 psi_sym=1 - sp.exp(-n), SC_G=-5.0 / 3.0, tau=1.0)
+        sim.InitRadialInterface(n_g=0.13, n_l=1.9, R=R)
+        sim.MainLoop([0, 20000])
+        sim.ComputeMomentsMeta()
+        nh = sim.sims_idpy_memory['n'].D2H()
+        mass, cm = sim.ComputeCenterOfMass()
+        ys, xs = np.divmod(np.arange(96 * 96), 96)
+        assert abs(mass - nh.sum()) <= 1e-12 * nh.sum()
+        assert abs(cm[0] - (nh * xs).sum() / nh.sum()) <= 1e-10
+        assert abs(cm[1] - (nh * ys).sum() / nh.sum()) <= 1e-10
+        sigma, R_eq, dP = sim.SurfaceTensionLaplace()
+        print('R', R, 'R_eq', R_eq, 'dP', dP, 'sigma', sigma, flush=True)
+        assert dP > 0 and abs(R_eq - R) < 2.0
+        out.append(sigma)
+        sim.End()
+    assert abs(out[0] - out[1]) <= 0.05 * out[1], out
+
+
+def test_dump_pop_snapshot(tmp_path):
+    """DumpPopSnapshot of a running simulation holds its device arrays and
+    scalar variables under ShanChenMultiPhase/... (IdpySims.py:73-136)."""
+    from idmx import configs
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=(64, 48), xi_stencil=XIStencils['D2Q9'],
+                             f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=-5.0 / 3.0, tau=1.0)
+    sim.SetDensityField(configs.droplet((64, 48), 10.0))
+    sim.MainLoop([0, 50])
+    fn = str(tmp_path / 'sim.hdf5')
+    sim.DumpPopSnapshot(file_name=fn)
+    c = 'ShanChenMultiPhase'
+    assert np.array_equal(sim.ReadSnapshotData(fn, c + '/idpy_memory/pop'),
+                          sim.sims_idpy_memory['pop'].D2H())
+    assert np.array_equal(sim.ReadSnapshotData(fn, c + '/idpy_memory/n'),
+                          sim.sims_idpy_memory['n'].D2H())
+    assert int(sim.ReadSnapshotData(fn, c + '/vars/V')) == 64 * 48
+    sim.End()
