@@ -181,9 +181,13 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     out['train_data_s'] = time.perf_counter() - t0
     log('training fields: %.1f s, converged %d/%d' % (out['train_data_s'],
                                                        res_tr.converged.sum(), n_train))
-    X = np.stack([warmstart.input_image(p.G, p.R, p.L)[0] for p in train_pts])
-    Y = np.stack([warmstart.target_image(f, p.L) - warmstart.input_image(p.G, p.R, p.L)[1]
-                  for f, p in zip(f_tr, train_pts)]).astype(np.float32)
+    keep = [i for i in range(n_train) if res_tr.converged[i] and np.isfinite(f_tr[i]).all()]
+    out['train_fields_used'] = len(keep)
+    X = np.stack([warmstart.input_image(train_pts[i].G, train_pts[i].R, train_pts[i].L)[0]
+                  for i in keep])
+    Y = np.stack([warmstart.target_image(f_tr[i], train_pts[i].L)
+                  - warmstart.input_image(train_pts[i].G, train_pts[i].R, train_pts[i].L)[1]
+                  for i in keep]).astype(np.float32)
     t0 = time.perf_counter()
     net, hist = warmstart.train(tenet, X, Y, epochs=epochs, batch=32, log=log)
     out['train_s'] = time.perf_counter() - t0
@@ -217,16 +221,86 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     out['start_err_cold'] = float(np.median([err0(sweep.cold_start(p), f)
                                              for p, f in zip(eval_pts, f_p)]))
     out['start_err_warm'] = float(np.median([err0(st, f) for st, f in zip(starts, f_p)]))
+    # Laplace pressures against the plain runs; droplets with R >= L / pi
+    # (interface longer than a stripe's 2L) can leave the droplet branch on
+    # a rounding-seeded asymmetry, so their end state is not unique
     dp = lambda f, p: sweep.laplace_dp(f, p.G)
+    stable = [p.R < p.L / np.pi for p in eval_pts]
+    out['n_eval_R_below_L_over_pi'] = int(sum(stable))
     for key in runs:
-        rel = [abs(dp(fw, p) - dp(fp, p)) / abs(dp(fp, p))
-               for fw, fp, p in zip(fields[key], f_p, eval_pts)]
+        rel = np.array([abs(dp(fw, p) - dp(fp, p)) / abs(dp(fp, p))
+                        for fw, fp, p in zip(fields[key], f_p, eval_pts)])
         out[key]['laplace_rel_diff_max'] = float(np.max(rel))
+        out[key]['laplace_rel_diff_max_R_below_L_over_pi'] = float(np.max(rel[stable]))
     for key in runs:
         out['speedup_%s_vs_plain' % key] = wall_p / out[key]['wall_s']
     out['speedup_warm_vs_plain'] = max(out['speedup_batched_warm_vs_plain'],
                                        out['speedup_batched_warm_fp64_vs_plain'])
     net.free()
+    tenet.End()
+    return out
+
+
+def run_precision_study(steps=5000):
+    """FP64 vs FP32-shifted populations on the D3Q19 MRT 256^3 droplet
+    (sphere R = 40, SC_G = -5/3) after `steps` steps: liquid and gas
+    densities (n at the centre and at site V-1, the Laplace probes), Laplace
+    pressure, surface tension from Laplace's law with the equimolar radius
+    (sigma = dP R / 2), maximum Mach number of the spurious currents, mass
+    drift, and the largest density difference between the two runs. The FP64
+    path is the one pinned to the oracle."""
+    from idmx import configs
+    from idmx.IdpyCode import IdpyMemory, Tenet
+    from idmx.LBM import hermite
+    from idmx.LBM._base import PopArray
+    from idmx.LBM._engine import SCEngine
+    w = WORKLOADS['d3q19_mrt_256']
+    tenet = Tenet(0)
+    n0 = configs.sphere3d()
+    mrt = hermite.mrt_kernel_tables(w['xi'], mrt_S(w['xi']))
+    G = configs.SC_G
+    psi = lambda x: 1.0 - np.exp(-x)
+    press = lambda x: x / 3.0 + 0.5 * G * psi(x) ** 2
+    out, fields = {}, {}
+    for prec in ('fp64', 'fp32s'):
+        eng = SCEngine(tenet, w['xi'], w['es'], w['dims'], 'exp', G, 1.0, precision=prec,
+                       rho0=1.0 if prec == 'fp32s' else 0.0, mrt=mrt)
+        V, D = eng.V, eng.D
+        n = IdpyMemory.OnDevice(n0, tenet)
+        ps = IdpyMemory.Zeros(V, np.float64, tenet)
+        u = IdpyMemory.Zeros(V * D, np.float64, tenet)
+        pop, swap = PopArray(eng, tenet), PopArray(eng, tenet)
+        eng.init_pop(pop, n, u)
+        m0, _ = eng.mass_momentum(pop)
+        t0 = time.perf_counter()
+        if eng.run(pop, swap, steps):
+            pop, swap = swap, pop
+        tenet.Sync()
+        wall = time.perf_counter() - t0
+        m1, _ = eng.mass_momentum(pop)
+        eng.moments(pop, n, ps, u)
+        mass = IdpyMemory.Sum(n)
+        nh = n.D2H()
+        u2 = (u.D2H().reshape(D, V) ** 2).sum(axis=0)
+        c = 128 + 256 * (128 + 256 * 128)
+        n_l, n_g = float(nh[c]), float(nh[V - 1])
+        dP = press(n_l) - press(n_g)
+        R_eq = (3.0 * (mass - V * n_g) / (4.0 * np.pi * (n_l - n_g))) ** (1.0 / 3.0)
+        out[prec] = {'n_l': n_l, 'n_g': n_g, 'dP': dP, 'R_eq': R_eq, 'sigma': 0.5 * dP * R_eq,
+                     'max_mach': float(np.sqrt(u2.max() * 3.0)),
+                     'mass_rel_drift': abs(m1 - m0) / m0, 'wall_s': wall}
+        fields[prec] = nh
+        for x in (n, ps, u, pop, swap):
+            x.Free()
+        eng.close()
+    a, b = out['fp64'], out['fp32s']
+    out['fp32s_vs_fp64'] = {
+        'max_abs_dn': float(np.abs(fields['fp64'] - fields['fp32s']).max()),
+        'rel_n_l': abs(b['n_l'] - a['n_l']) / a['n_l'],
+        'rel_n_g': abs(b['n_g'] - a['n_g']) / a['n_g'],
+        'rel_sigma': abs(b['sigma'] - a['sigma']) / abs(a['sigma']),
+        'rel_max_mach': abs(b['max_mach'] - a['max_mach']) / a['max_mach']}
+    out['steps'] = steps
     tenet.End()
     return out
 
@@ -279,13 +353,18 @@ def tune_report(names):
 
 
 def traffic_from_profiles(name):
+    """HBM bytes per launch of the workload's fused kernel from the committed
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json, scaled by the
+    calibration copy), with the launch shape it was measured at; None if
+    the workload was not profiled."""
     p = os.path.join(ROOT, 'profiles', 'traffic.json')
-    if os.path.exists(p):
-        try:
-            return json.load(open(p)).get(name)
-        except Exception:
-            return None
-    return None
+    try:
+        t = json.load(open(p)).get(name)
+    except (OSError, ValueError):
+        return None, None
+    if not t:
+        return None, None
+    return float(t['bytes_per_launch']), t.get('launch')
 
 
 def main():
@@ -305,6 +384,9 @@ def main():
                     help='time the plain copy kernel (bandwidth / counter calibration)')
     ap.add_argument('--tune-report', default=None,
                     help='print the autotuner table for these comma-separated workloads')
+    ap.add_argument('--precision-study', type=int, default=0, metavar='STEPS',
+                    help='FP64 vs FP32-shifted observables on the D3Q19 MRT 256^3 droplet '
+                         'after STEPS steps')
     ap.add_argument('--sweep', action='store_true',
                     help='run the (G, R, L) sweep study (time to steady state, learned '
                          'warm start vs plain) instead of the step benchmark')
@@ -320,6 +402,10 @@ def main():
     rank = int(os.environ.get('RANK', '0'))
     if args.calibrate:
         print(json.dumps({'calibration': calibrate()}), flush=True)
+        return
+    if args.precision_study:
+        print(json.dumps({'precision_study': run_precision_study(args.precision_study)}),
+              flush=True)
         return
     if args.tune_report:
         tune_report(args.tune_report.split(','))
@@ -338,6 +424,7 @@ def main():
     launch = tuple(int(v) for v in args.launch.split(',')) if args.launch else None
     r = run_gpu(name, args.steps, args.warmup, tune=not args.no_tune, launch=launch)
     achieved = r['B_alg'] * r['V'] / (r['kernel_ms'] / 1e3) / 1e9
+    traffic, traffic_launch = traffic_from_profiles(name)
     line = {
         'metric': 'MLUPS & %HBM roofline (D3Q19 SC)',
         'value': round(r['mlups'], 1),
@@ -354,7 +441,7 @@ def main():
         'config': workload_desc(name, w),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
                      'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4),
-                     'traffic': traffic_from_profiles(name),
+                     'traffic': traffic, 'traffic_launch': traffic_launch,
                      'bytes_per_site': r['B_alg'], 'kernel_ms': round(r['kernel_ms'], 4)},
         'mass_rel_drift': r['mass_rel_drift'],
         'launch': r['launch'],

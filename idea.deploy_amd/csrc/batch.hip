@@ -417,9 +417,10 @@ int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src, int flags) {
   return IDMX_OK;
 }
 
-int idmx_batch_symmetrize(idmx_batch_t b, int dom) {
+int idmx_batch_symmetrize(idmx_batch_t b, int dom, const double* target_mass) {
   BCHECK(b && dom >= -1 && dom < b->ndom, "idmx_batch_symmetrize: invalid argument");
   hipStream_t st = b->ctx->stream;
+  idmx_ctx_s* c = b->ctx;
   for (int d = dom < 0 ? 0 : dom; d < (dom < 0 ? b->ndom : dom + 1); d++) {
     if (dom < 0 && !b->active[d]) continue;
     const BatchDom& D = b->doms[d];
@@ -427,17 +428,33 @@ int idmx_batch_symmetrize(idmx_batch_t b, int dom) {
     const unsigned grid = (unsigned)std::min<int64_t>((V + 255) / 256, 4096);
     void* pop = (char*)b->pop[b->cur] + b->elem * D.pop_off;
     double* nref = b->n_prev + D.site_off;
-    if (b->prec == IDMX_PREC_F64) {
+    if (b->prec == IDMX_PREC_F64)
       hipLaunchKernelGGL((k_batched_density<double, false>), dim3(grid), dim3(256), 0, st,
                          (const double*)pop, b->stage, V, 0.0);
-      hipLaunchKernelGGL((k_batched_symmetrize<double, false>), dim3(grid), dim3(256), 0, st,
-                         b->stage, (double*)pop, D.L0, D.L1, 0.0, nref);
-    } else {
+    else
       hipLaunchKernelGGL((k_batched_density<float, true>), dim3(grid), dim3(256), 0, st,
                          (const float*)pop, b->stage, V, D.rho0);
-      hipLaunchKernelGGL((k_batched_symmetrize<float, true>), dim3(grid), dim3(256), 0, st,
-                         b->stage, (float*)pop, D.L0, D.L1, D.rho0, nref);
+    IDMX_HIP(hipGetLastError());
+    double shift = 0.0;
+    if (target_mass) {
+      const unsigned rg = (unsigned)std::min<int64_t>((V + 255) / 256, c->partials_cap);
+      hipLaunchKernelGGL(k_batched_sum_partial, dim3(rg), dim3(256), 0, st, b->stage, V,
+                         c->partials);
+      IDMX_HIP(hipGetLastError());
+      std::vector<double> h(rg);
+      IDMX_HIP(hipMemcpyAsync(h.data(), c->partials, sizeof(double) * rg,
+                              hipMemcpyDeviceToHost, st));
+      IDMX_HIP(hipStreamSynchronize(st));
+      double m = 0.0;
+      for (double v : h) m += v;
+      shift = (target_mass[d] - m) / (double)V;
     }
+    if (b->prec == IDMX_PREC_F64)
+      hipLaunchKernelGGL((k_batched_symmetrize<double, false>), dim3(grid), dim3(256), 0, st,
+                         b->stage, (double*)pop, D.L0, D.L1, 0.0, shift, nref);
+    else
+      hipLaunchKernelGGL((k_batched_symmetrize<float, true>), dim3(grid), dim3(256), 0, st,
+                         b->stage, (float*)pop, D.L0, D.L1, D.rho0, shift, nref);
     IDMX_HIP(hipGetLastError());
   }
   IDMX_HIP(hipStreamSynchronize(st));

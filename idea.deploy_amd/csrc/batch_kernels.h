@@ -328,14 +328,14 @@ k_batched_rest(const TI* __restrict__ src, TO* __restrict__ dst, int64_t V, doub
 // centre c = (L0/2, L1/2) (integer division): the average over the mirror
 // images x -> 2 c0 - x, y -> 2 c1 - y (mod L) and, on square domains, the
 // transpose, i.e. the dihedral group of the square (8 images) or of the
-// rectangle (4). The populations become the rest equilibrium of the
-// projected density. n holds the current density (read only); the result
-// goes to pop and to the check reference n_ref. Averaging over a permutation
-// group of the sites keeps the total mass.
+// rectangle (4), plus a uniform shift (0 keeps the mass: averaging over a
+// permutation group of the sites does). The populations become the rest
+// equilibrium of the result. n holds the current density (read only); the
+// result goes to pop and to the check reference n_ref.
 template <class T, bool SHIFT>
 __global__ void __launch_bounds__(256)
 k_batched_symmetrize(const double* __restrict__ n, T* __restrict__ pop, int L0, int L1,
-                     double rho0, double* __restrict__ n_ref) {
+                     double rho0, double shift, double* __restrict__ n_ref) {
   const int64_t V = (int64_t)L0 * L1;
   const int c0 = L0 / 2, c1 = L1 / 2;
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
@@ -350,12 +350,23 @@ k_batched_symmetrize(const double* __restrict__ n, T* __restrict__ pop, int L0, 
            n[(int64_t)xr * L0 + yr];
       m = 8.0;
     }
-    const double v = s / m;
+    const double v = s / m + shift;
 #pragma unroll
     for (int q = 0; q < 9; q++)
       pop[(size_t)q * V + g] = (T)(D2Q9::w[q] * (SHIFT ? v - rho0 : v));
     n_ref[g] = v;
   }
+}
+
+// Per-block partial sums of n over V sites (FP64).
+__global__ void __launch_bounds__(256)
+k_batched_sum_partial(const double* __restrict__ n, int64_t V, double* __restrict__ out) {
+  double s = 0.0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
+       g += (int64_t)gridDim.x * blockDim.x)
+    s += n[g];
+  s = block_reduce<0>(s);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 }  // namespace idmx

@@ -117,10 +117,17 @@ class Batch:
         _lib.call('idmx_batch_check', self.handle, out.ctypes.data)
         return out
 
-    def symmetrize(self, dom=-1):
+    def symmetrize(self, dom=-1, mass=None):
         """Projects domain dom's density (-1: every live domain) onto the
-        fields symmetric about its centre; populations to rest equilibrium."""
-        _lib.call('idmx_batch_symmetrize', self.handle, int(dom))
+        fields symmetric about its centre, then (mass: one total per domain)
+        shifts it uniformly to that mass; populations to rest equilibrium."""
+        m = None
+        if mass is not None:
+            m = np.ascontiguousarray(np.asarray(mass, dtype=np.float64))
+            if m.size != self.n:
+                raise ValueError('symmetrize: one mass per domain')
+        _lib.call('idmx_batch_symmetrize', self.handle, int(dom),
+                  None if m is None else m.ctypes.data)
 
     def copy_state_from(self, other, rest=False):
         """Populations of `other` (converted); rest=True writes the rest
@@ -198,10 +205,17 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     n = len(points)
     steps = np.zeros(n, dtype=np.int64)
     hist = []
-    fb_all = np.zeros(n, dtype=bool)
     fail_all = np.zeros(n, dtype=bool)
     warm = np.array([start is not None and start[i] is not None for i in range(n)])
-    init = [cold_start(p) if not warm[i] else start[i] for i, p in enumerate(points)]
+    init = [cold_start(p) if not warm[i] else np.asarray(start[i], dtype=np.float64).ravel()
+            for i, p in enumerate(points)]
+    fb_all = np.zeros(n, dtype=bool)
+    for i, p in enumerate(points):
+        if warm[i] and not np.isfinite(init[i]).all():   # unusable warm start
+            init[i] = cold_start(p)
+            warm[i] = False
+            fb_all[i] = True
+    mass0 = np.array([float(np.sum(a)) for a in init])
     t0 = time.perf_counter()
     b64 = Batch(tenet, points, 'fp64', estencil)
     if launch:
@@ -209,7 +223,9 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     gpu_ms = 0.0
 
     def restart64(i):
-        b64.set_density(i, cold_start(points[i]))
+        n0 = cold_start(points[i])
+        mass0[i] = float(np.sum(n0))
+        b64.set_density(i, n0)
 
     if mode == 'fp32+fp64':
         b32 = Batch(tenet, points, 'fp32s', estencil, rho0=rho0)
@@ -221,7 +237,9 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
                 b32.symmetrize(i)
 
         def restart32(i):
-            b32.set_density(i, cold_start(points[i]))
+            n0 = cold_start(points[i])
+            mass0[i] = float(np.sum(n0))
+            b32.set_density(i, n0)
 
         _, failed, fb, ms = _relax(b32, np.ones(n, dtype=bool), warm, steps, tol32, check_every,
                                    max_steps, grow_checks, hist, restart32, 'fp32',
@@ -233,8 +251,10 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
         # only, at rest
         b64.copy_state_from(b32, rest=True)
         b32.close()
+        # FP32 storage drifts the mass (~1e-5 relative over a few thousand
+        # steps), which would move the steady state: restore it
         if symmetric:
-            b64.symmetrize(-1)
+            b64.symmetrize(-1, mass=mass0)
         for i in np.nonzero(failed)[0]:   # FP32 blew up: FP64 from the cold start
             restart64(i)
             fb_all[i] = True

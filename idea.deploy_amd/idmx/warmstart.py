@@ -303,6 +303,8 @@ class UNet:
 def train(tenet, inputs, targets, epochs=200, batch=32, lr=2e-3, seed=0, log=None):
     """inputs (M, S, S, CIN) float32, targets (M, S, S) float32 residuals.
     Returns the trained UNet and the loss history (one value per epoch)."""
+    if not (np.isfinite(inputs).all() and np.isfinite(targets).all()):
+        raise ValueError('train: non-finite inputs or targets')
     rng = np.random.default_rng(seed)
     net = UNet(tenet, batch, seed=seed)
     M = inputs.shape[0]

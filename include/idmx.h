@@ -267,13 +267,15 @@ int idmx_batch_check(idmx_batch_t b, double* dmax);
 int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src, int flags);
 /* Projects the density of domain dom (-1: every live domain) onto the
  * fields symmetric about its centre (L0/2, L1/2): the average over the
- * mirror images in x and y and, for square domains, the transpose. The
- * populations become the rest equilibrium of the projected density and it
- * becomes the check reference; the total mass is unchanged. For centred
- * droplets, whose steady state has this symmetry: FP32 rounding or an
- * approximate start breaks it, and an off-centre droplet drifts towards a
- * symmetric lattice position over ~1e5 steps. */
-int idmx_batch_symmetrize(idmx_batch_t b, int dom);
+ * mirror images in x and y and, for square domains, the transpose. With
+ * target_mass (ndom values, or NULL to keep the mass) a uniform shift then
+ * sets the domain's total mass to target_mass[d]. The populations become
+ * the rest equilibrium of the result and it becomes the check reference.
+ * For centred droplets, whose steady state has this symmetry: FP32 rounding
+ * or an approximate start breaks it (an off-centre droplet drifts towards a
+ * symmetric lattice position over ~1e5 steps), and FP32 storage drifts the
+ * mass by ~1e-5 relative over a few thousand steps. */
+int idmx_batch_symmetrize(idmx_batch_t b, int dom, const double* target_mass);
 int idmx_batch_last_run_ms(idmx_batch_t b, double* ms);
 
 /* ---- warm-start network kernels ---------------------------------------
