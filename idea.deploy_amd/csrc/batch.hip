@@ -160,9 +160,11 @@ int idmx_batch_create(idmx_ctx_t ctx, const idmx_batch_domain* doms, int ndom, i
   int64_t po = 0, so = 0;
   for (int d = 0; d < ndom; d++) {
     const idmx_batch_domain& s = doms[d];
-    if (s.L0 < 2 * R + 1 || s.L1 < 2 * R + 1 || s.L0 > 65536 || s.L1 > 65536 || s.tau <= 0) {
+    if (s.L0 < 2 * R + 1 || s.L1 < 2 * R + 1 || s.L0 > 65536 || s.L1 > 65536 || s.tau <= 0 ||
+        (int64_t)s.L0 * s.L1 * (int64_t)b->elem >= ((int64_t)1 << 32)) {
       delete b;
-      set_error("idmx_batch_create: domain sizes must be in [2R+1, 65536], tau > 0");
+      set_error("idmx_batch_create: domain sizes must be in [2R+1, 65536] with "
+                "L0 L1 sizeof(population) < 4 GiB, tau > 0");
       return IDMX_E_ARG;
     }
     BatchDom D{};

@@ -344,7 +344,10 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
               "idmx_sc_create: each lattice size must be >= 2*reach+1 and <= 2^20");
   if (D == 2) CHECK_ARG(d->dims[2] <= 1, "idmx_sc_create: dims[2] must be 1 for D2Q9");
   const int64_t V = (int64_t)d->dims[0] * d->dims[1] * (D == 3 ? d->dims[2] : 1);
-  CHECK_ARG(V * Q < ((int64_t)1 << 40), "idmx_sc_create: lattice too large");
+  // one population stream (V values) must stay below 4 GiB: the kernels
+  // address a site by a 32-bit byte offset from the stream's base
+  CHECK_ARG(V * (d->precision == IDMX_PREC_F64 ? 8 : 4) < ((int64_t)1 << 32),
+            "idmx_sc_create: lattice too large (V * sizeof(population) must be < 4 GiB)");
   if (d->collision == IDMX_MRT)
     CHECK_ARG(d->mrt_diag || (d->mrt_A && d->mrt_beta && d->mrt_gamma),
               "idmx_sc_create: MRT tables missing");
@@ -443,8 +446,9 @@ int idmx_sc_set_launch(idmx_sc_t s, int tx, int ty, int H, int hist) {
     CHECK_ARG((tx == 64 || tx == 128 || tx == 256) && ty == 1,
               "idmx_sc_set_launch: 2D tiles are 64, 128 or 256 sites wide, ty = 1");
   else
-    CHECK_ARG((tx == 32 && (ty == 8 || ty == 16)) || (tx == 64 && (ty == 4 || ty == 8)),
-              "idmx_sc_set_launch: 3D tiles are 32x8, 32x16, 64x4 or 64x8");
+    CHECK_ARG((tx == 32 && ty == 8) || (tx == 64 && ty == 4) || (tx == 128 && ty == 2) ||
+                  (tx == 256 && ty == 2),
+              "idmx_sc_set_launch: 3D tiles are 32x8, 64x4, 128x2 or 256x2");
   CHECK_ARG(H >= 1, "idmx_sc_set_launch: H must be >= 1");
   s->tx = tx;
   s->ty = ty;

@@ -86,8 +86,8 @@ struct Impl {
       } else {
         switch (s->tx * 100 + s->ty) {
           case 6404: return variant<64, 4>(s, pop, swap);
-          case 3216: return variant<32, 16>(s, pop, swap);
-          case 6408: return variant<64, 8>(s, pop, swap);
+          case 12802: return variant<128, 2>(s, pop, swap);
+          case 25602: return variant<256, 2>(s, pop, swap);
           default: return variant<32, 8>(s, pop, swap);
         }
       }

@@ -277,6 +277,25 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
                        gpu_ms=gpu_ms, history=hist), fields
 
 
+def is_centred_droplet(n, tol=2.0):
+    """True when an L x L density field is a disc of liquid centred on
+    (L/2, L/2): liquid at the centre, gas at the corner, and the liquid
+    chord through the centre along x, y and both diagonals within `tol`
+    lattice units of the equimolar diameter."""
+    n = np.asarray(n).reshape(int(round(np.sqrt(np.size(n)))), -1)
+    L = n.shape[0]
+    c = L // 2
+    n_l, n_g = n[c, c], n[-1, -1]
+    if not n_l > n_g:
+        return False
+    mid = 0.5 * (n_l + n_g)
+    R = np.sqrt(max(n.sum() - n.size * n_g, 0.0) / (np.pi * (n_l - n_g)))
+    k = np.arange(L)
+    chords = [(n[c, :] > mid).sum(), (n[:, c] > mid).sum(),
+              np.sqrt(2) * (n[k, k] > mid).sum(), np.sqrt(2) * (n[k, L - 1 - k] > mid).sum()]
+    return all(abs(ch - 2 * R) <= 2 * tol for ch in chords)
+
+
 def laplace_dp(n, G, psi=lambda x: 1 - np.exp(-x), c2=1 / 3, e2=1.0):
     """p(centre) - p(corner) with p = n c2 + G e2 psi(n)^2 / 2
     (idpy/LBM/MultiPhase.py:1690-1768)."""

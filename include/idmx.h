@@ -203,7 +203,7 @@ int idmx_sc_pop_to_f64(idmx_sc_t sim, const void* pop, double* dst_dev);
 int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
 
 /* Launch shape of the fused step: tile tx x ty (2D: tx = 64, 128 or 256
- * sites of one row, ty = 1; 3D: 32x8, 32x16, 64x4, 64x8), H planes marched
+ * sites of one row, ty = 1; 3D: 32x8, 64x4, 128x2, 256x2), H planes marched
  * per block, and the variant `hist`: 1 holds the tile's populations in
  * registers between their read and their collision, 0 re-reads them, 2
  * re-reads them and prefetches the next plane before the collision, 3 and 4
