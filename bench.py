@@ -218,9 +218,10 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
                                                            res.converged.sum()))
     # initial-field error of warm vs cold start against the converged fields
     err0 = lambda st, f: float(np.abs(st.ravel() - f.ravel()).max())
-    out['start_err_cold'] = float(np.median([err0(sweep.cold_start(p), f)
-                                             for p, f in zip(eval_pts, f_p)]))
-    out['start_err_warm'] = float(np.median([err0(st, f) for st, f in zip(starts, f_p)]))
+    ok = [i for i in range(n_eval) if conv_p[i]]
+    out['start_err_cold'] = float(np.median([err0(sweep.cold_start(eval_pts[i]), f_p[i])
+                                             for i in ok]))
+    out['start_err_warm'] = float(np.median([err0(starts[i], f_p[i]) for i in ok]))
     # Laplace pressures against the plain runs, over the points whose end
     # state is a centred droplet in both runs. Near the droplet / stripe
     # transition (R_eq ~ L / pi) a symmetric droplet can leave the droplet

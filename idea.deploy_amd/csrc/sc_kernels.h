@@ -212,6 +212,40 @@ __device__ inline int64_t lex(const Geom& g, int x, int y, int z) {
   return (int64_t)x + (int64_t)g.L0 * ((int64_t)y + (int64_t)g.L1 * z);
 }
 
+// Population access of the fused step. pop_ld(p, q, V, g) is p[q V + g].
+// For 4-byte populations the base of population q (the same for every lane)
+// is pinned to SGPRs and the site is a 32-bit byte offset, so the compiler
+// emits global_load v, v_off, s[base:base+1] with one offset register shared
+// by the Q populations of a site; measured faster for FP32 storage. For
+// 8-byte populations the plain 64-bit form measured faster (the FP64 MRT
+// step is limited by its other registers, and the SGPR form changed its
+// schedule for the worse), so it is kept. The 32-bit offset needs
+// V sizeof(T) < 2^32, which simulation and batch creation check.
+#define IDMX_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ uint64_t sgpr_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+template <class T>
+__device__ __forceinline__ T pop_ld(const T* __restrict__ p, int q, int64_t V, int64_t g) {
+  if constexpr (sizeof(T) == 4) {
+    const IDMX_GLOBAL char* b = (const IDMX_GLOBAL char*)sgpr_u64((uint64_t)(p + (size_t)q * V));
+    return *(const IDMX_GLOBAL T*)(b + (uint32_t)g * (uint32_t)sizeof(T));
+  } else {
+    return p[(size_t)q * V + g];
+  }
+}
+template <class T>
+__device__ __forceinline__ void pop_st_nt(T* __restrict__ p, int q, int64_t V, int64_t g, T v) {
+  if constexpr (sizeof(T) == 4) {
+    IDMX_GLOBAL char* b = (IDMX_GLOBAL char*)sgpr_u64((uint64_t)(p + (size_t)q * V));
+    __builtin_nontemporal_store(v, (IDMX_GLOBAL T*)(b + (uint32_t)g * (uint32_t)sizeof(T)));
+  } else {
+    __builtin_nontemporal_store(v, p + (size_t)q * V + g);
+  }
+}
+
 // Push-stream one site's f* to its neighbours (periodic).
 template <class S, class T>
 __device__ inline void push_store(T* __restrict__ dst, const Geom& geo, int x,
@@ -221,7 +255,7 @@ __device__ inline void push_store(T* __restrict__ dst, const Geom& geo, int x,
     const int xt = S::cx[q] ? wrap(x + S::cx[q], geo.L0) : x;
     const int yt = S::cy[q] ? wrap(y + S::cy[q], geo.L1) : y;
     const int zt = S::cz[q] ? wrap(z + S::cz[q], geo.L2) : z;
-    __builtin_nontemporal_store(fo[q], dst + (size_t)q * geo.V + lex(geo, xt, yt, zt));
+    pop_st_nt(dst, q, geo.V, lex(geo, xt, yt, zt), fo[q]);
   }
 }
 
@@ -332,7 +366,7 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
   if (PF) {
     const int64_t b0 = (int64_t)(((m0 - R) % Lm + Lm) % Lm) * PS;
 #pragma unroll
-    for (int q = 0; q < Q; q++) fn[q] = pop[(size_t)q * geo.V + b0 + own_off];
+    for (int q = 0; q < Q; q++) fn[q] = pop_ld(pop, q, geo.V, b0 + own_off);
   }
 
   for (int j = 0; j < Hs + 2 * R; j++) {
@@ -346,7 +380,7 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
         for (int q = 0; q < Q; q++) fh[HR - 1][q] = fn[q];
       } else {
 #pragma unroll
-        for (int q = 0; q < Q; q++) fh[HR - 1][q] = pop[(size_t)q * geo.V + base + own_off];
+        for (int q = 0; q < Q; q++) fh[HR - 1][q] = pop_ld(pop, q, geo.V, base + own_off);
       }
 #pragma unroll
       for (int q = 0; q < Q; q++) s += fh[HR - 1][q];
@@ -358,7 +392,7 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
       if (tid + k * NT < NH) {
         T s = 0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) s += pop[(size_t)q * geo.V + base + hoff[k]];
+        for (int q = 0; q < Q; q++) s += pop_ld(pop, q, geo.V, base + hoff[k]);
         lds[slot][hys[k]][hxs[k]] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
       }
     }
@@ -366,7 +400,7 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
     if (PF && j + 1 < Hs + 2 * R) {
       const int64_t bn = (int64_t)(((m0 - R + j + 1) % Lm + Lm) % Lm) * PS;
 #pragma unroll
-      for (int q = 0; q < Q; q++) fn[q] = pop[(size_t)q * geo.V + bn + own_off];
+      for (int q = 0; q < Q; q++) fn[q] = pop_ld(pop, q, geo.V, bn + own_off);
     }
     if (j >= 2 * R && own_valid) {
       // plane with lead index j - R
@@ -375,7 +409,7 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
         const int64_t cbase = (int64_t)mc * PS;
         T s = 0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) fh[0][q] = pop[(size_t)q * geo.V + cbase + own_off];
+        for (int q = 0; q < Q; q++) fh[0][q] = pop_ld(pop, q, geo.V, cbase + own_off);
 #pragma unroll
         for (int q = 0; q < Q; q++) s += fh[0][q];
         dnh[0] = s;

@@ -14,9 +14,10 @@ Modes
   FP32 + FP64: any start, FP32 shifted populations to tol32, then the state
                is converted to FP64 and relaxed to tol (FP64 fixes the final
                accuracy)
-A caller-supplied start whose residual stays above twice its best value for
-`grow_checks` consecutive checks (or turns non-finite) is reset to the cold
-start once (the fallback of a bad warm start). A cold start is never reset:
+A caller-supplied start whose residual stays above ten times its best value
+for `grow_checks` consecutive checks (or turns non-finite) is reset to the
+cold start once (the fallback of a bad warm start; the relaxation of a good
+start is not monotone either, so a smaller factor throws away good starts). A cold start is never reset:
 its trajectory would repeat.
 """
 import ctypes as C
@@ -139,8 +140,8 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
            restart, label, plateau=False):
     """Runs the live domains of `batch` until each one's max |dn| < tol, or
     until every live domain has run max_steps steps in this call. A domain
-    on a caller-supplied start (warm[i]) whose residual stays above twice
-    its best value for `grow_checks` consecutive checks, or turns
+    on a caller-supplied start (warm[i]) whose residual stays above ten
+    times its best value for `grow_checks` consecutive checks, or turns
     non-finite, is handed to restart(i) (the cold-start fallback) once. A
     cold domain that turns non-finite is dropped as failed. With
     plateau=True (the FP32 phase) a domain also finishes once its residual
@@ -164,7 +165,7 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
         ran[active] += check_every
         dn = batch.check()
         fin = np.isfinite(dn)
-        grows = np.where(active & fin & (dn > 2 * best) & (dn > tol), grows + 1, 0)
+        grows = np.where(active & fin & (dn > 10 * best) & (dn > tol), grows + 1, 0)
         stale = np.where(active & fin & (dn > 0.8 * best), stale + 1, 0)
         best = np.where(active & fin, np.minimum(best, dn), best)
         done = active & fin & (dn < tol)
@@ -322,7 +323,9 @@ def sample_points(count, seed, Ls=tuple(range(64, 513, 32)), G_lit=(-5.5, -4.2),
 def plain_runs(tenet, points, tol=1e-10, check_every=200, max_steps=400000, starts=None):
     """The unbatched baseline: one FP64 simulation per point, run the way a
     notebook runs MainLoop with a convergence callback, i.e. checkpoints of
-    `check_every` steps and a device max |n - n_prev| check after each.
+    `check_every` steps and a device max |n - n_prev| check after each. A
+    run whose residual turns non-finite stops there (as the batched sweep
+    drops such a domain), so both spend the same steps on diverging points.
     Returns (steps, converged, wall seconds, final densities)."""
     import time as _t
     from .IdpyCode import IdpyMemory
@@ -351,6 +354,8 @@ def plain_runs(tenet, points, tol=1e-10, check_every=200, max_steps=400000, star
             n, nprev = nprev, n
             if d < tol:
                 conv[i] = True
+                break
+            if not np.isfinite(d):
                 break
         fields.append(nprev.D2H().reshape(p.L, p.L))
         for x in (n, nprev, psi, u, a, b):
