@@ -170,14 +170,20 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     t0 = time.perf_counter()
     l64, tim64 = tune.tune_batch(tenet, eval_pts, 'fp64', use_cache=False)
     l32, tim32 = tune.tune_batch(tenet, eval_pts, 'fp32s', use_cache=False)
+    s64 = tune.tune_batch_small(tenet, eval_pts, 'fp64')
+    s32 = tune.tune_batch_small(tenet, eval_pts, 'fp32s')
     out['tune_s'] = time.perf_counter() - t0
+    out['launch_small_fp64'] = {'best': s64[0], 'sites': s64[1]}
+    out['launch_small_fp32s'] = {'best': s32[0], 'sites': s32[1]}
     out['launch_fp64'] = {'best': l64, 'ms_per_step': tim64[','.join(map(str, l64))]}
     out['launch_fp32s'] = {'best': l32, 'ms_per_step': tim32[','.join(map(str, l32))]}
     log('batch launch fp64 %s %.4f ms/step, fp32s %s %.4f ms/step' % (
         l64, out['launch_fp64']['ms_per_step'], l32, out['launch_fp32s']['ms_per_step']))
     t0 = time.perf_counter()
     lt, _ = tune.tune_batch(tenet, train_pts, 'fp64', use_cache=False)
-    res_tr, f_tr = sweep.run_sweep(tenet, train_pts, mode='fp64', tol=1e-8, launch=lt)
+    lts = tune.tune_batch_small(tenet, train_pts, 'fp64')
+    res_tr, f_tr = sweep.run_sweep(tenet, train_pts, mode='fp64', tol=1e-8, launch=lt,
+                                   launch_small=lts)
     out['train_data_s'] = time.perf_counter() - t0
     log('training fields: %.1f s, converged %d/%d' % (out['train_data_s'],
                                                        res_tr.converged.sum(), n_train))
@@ -205,7 +211,7 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     fields = {}
     for key, (st, mode) in runs.items():
         res, f = sweep.run_sweep(tenet, eval_pts, start=st, mode=mode, launch=l64,
-                                 launch32=l32)
+                                 launch32=l32, launch_small=s64, launch32_small=s32)
         fields[key] = f
         wall = res.wall_s + (t_inf if st is not None else 0.0)
         out[key] = {'wall_s': wall, 'gpu_ms': res.gpu_ms, 'steps_mean': float(res.steps.mean()),

@@ -38,6 +38,20 @@ class IdpyArrayHIP:
         _lib.call('idmx_d2h', self.tenet.handle, out.ctypes.data, self.ptr, self.nbytes)
         return out
 
+    def D2HAt(self, indices):
+        """The values at the given flat indices, copied element by element
+        (for probes such as the Laplace pressure's two sites, without
+        copying the whole field)."""
+        out = np.empty(len(indices), dtype=self.dtype)
+        isz = self.dtype.itemsize
+        for k, i in enumerate(indices):
+            i = int(i)
+            if not 0 <= i < self.size:
+                raise IndexError('D2HAt: index %d out of range' % i)
+            _lib.call('idmx_d2h', self.tenet.handle, out[k:].ctypes.data,
+                      C.c_void_p(self.ptr.value + i * isz), isz)
+        return out
+
     def SetConst(self, const):
         self.H2D(np.full(self.shape, const, dtype=self.dtype))
 

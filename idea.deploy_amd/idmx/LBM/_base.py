@@ -476,8 +476,7 @@ class _ShanChenCommon(RootLB):
                                   self.sims_vars['dim_strides'])
         if outside is None:
             outside = self.sims_vars['V'] - 1
-        n = self.sims_idpy_memory['n'].D2H()
-        nio = np.array([n[int(inside)], n[int(outside)]])
+        nio = self.sims_idpy_memory['n'].D2HAt([inside, outside])
         self.sims_vars['n_in_n_out'] = nio
         p_in, p_out = self.PBulk(nio[0]), self.PBulk(nio[1])
         return p_in, p_out, p_in - p_out

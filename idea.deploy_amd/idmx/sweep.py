@@ -136,8 +136,21 @@ class Batch:
         _lib.call('idmx_batch_copy_state', self.handle, other.handle, 1 if rest else 0)
 
 
+def _apply_launch(batch, active, launch, launch_small):
+    """launch (the batch's tuned shape) while the live domains hold more
+    sites than launch_small = (shape, sites) was tuned for, then the small
+    shape: a launch tuned for a full, bandwidth-bound batch is latency-bound
+    once only a few domains remain."""
+    if launch_small is None:
+        return
+    live = sum(batch.points[i].L ** 2 for i in np.nonzero(active)[0])
+    want = launch_small[0] if live <= launch_small[1] else launch
+    if want is not None and tuple(want) != batch.get_launch()[:4]:
+        batch.set_launch(*want)
+
+
 def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks, hist,
-           restart, label, plateau=False):
+           restart, label, plateau=False, launch=None, launch_small=None):
     """Runs the live domains of `batch` until each one's max |dn| < tol, or
     until every live domain has run max_steps steps in this call. A domain
     on a caller-supplied start (warm[i]) whose residual stays above ten
@@ -147,6 +160,7 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
     plateau=True (the FP32 phase) a domain also finishes once its residual
     has not improved by 20 % for `grow_checks` checks: FP32 rounding sets a
     floor below which it cannot go, and the FP64 phase takes over there.
+    launch / launch_small: see _apply_launch.
     Returns (still active, failed, fell back, GPU ms)."""
     n = batch.n
     best = np.full(n, np.inf)
@@ -158,6 +172,7 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
     gpu_ms = 0.0
     active = active.copy()
     batch.set_active(active)
+    _apply_launch(batch, active, launch, launch_small)
     batch.check()  # reference densities
     while active.any():
         gpu_ms += batch.run(check_every)
@@ -188,17 +203,21 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
             break
         if changed:
             batch.set_active(active)
+            _apply_launch(batch, active, launch, launch_small)
     return active, failed, fell_back, gpu_ms
 
 
 def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
               check_every=200, max_steps=400000, grow_checks=5, estencil='E4',
-              launch=None, launch32=None, rho0=1.0, symmetric=True):
+              launch=None, launch32=None, rho0=1.0, symmetric=True, launch_small=None,
+              launch32_small=None):
     """Relax every point to steady state; returns SweepResult and the final
     FP64 densities. start: None (cold) or a list of initial densities (None
     entries start cold). mode: 'fp64' (FP64 only) or 'fp32+fp64'.
     max_steps bounds each phase. launch / launch32: (kind, tx, H, grid) of
-    the FP64 / FP32 batch (None keeps the library default).
+    the FP64 / FP32 batch (None keeps the library default); launch_small /
+    launch32_small: ((kind, tx, H, grid), sites), the shape used once the
+    live domains hold at most that many sites (tune.tune_batch_small).
     symmetric: the points are centred droplets, whose steady state is
     symmetric about the domain centre; warm starts and the FP32 -> FP64
     hand-over are projected onto symmetric fields (a broken symmetry makes
@@ -244,7 +263,7 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
 
         _, failed, fb, ms = _relax(b32, np.ones(n, dtype=bool), warm, steps, tol32, check_every,
                                    max_steps, grow_checks, hist, restart32, 'fp32',
-                                   plateau=True)
+                                   plateau=True, launch=launch32, launch_small=launch32_small)
         gpu_ms += ms
         fb_all |= fb
         # FP32 rounding leaves each domain a small net momentum that FP64
@@ -267,7 +286,8 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     else:
         raise ValueError('mode is fp64 or fp32+fp64')
     rem, failed, fb, ms = _relax(b64, np.ones(n, dtype=bool), warm, steps, tol, check_every,
-                                 max_steps, grow_checks, hist, restart64, 'fp64')
+                                 max_steps, grow_checks, hist, restart64, 'fp64',
+                                 launch=launch, launch_small=launch_small)
     gpu_ms += ms
     fb_all |= fb
     fail_all |= failed

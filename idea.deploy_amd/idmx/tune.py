@@ -111,3 +111,12 @@ def tune_batch(tenet, points, precision, estencil='E4', steps=20, use_cache=True
     best = tuple(int(v) for v in best.split(','))
     _store(key, {'best': best, 'timings': timings})
     return best, timings
+
+
+def tune_batch_small(tenet, points, precision, estencil='E4', count=3, steps=20):
+    """The launch for the end of a sweep, when only a few domains are left:
+    tuned on the `count` largest domains of `points`. Returns ((kind, tx, H,
+    grid), their total sites), the form run_sweep's launch_small takes."""
+    big = sorted(points, key=lambda p: p.L, reverse=True)[:count]
+    best, _ = tune_batch(tenet, big, precision, estencil, steps=steps, use_cache=False)
+    return best, sum(p.L * p.L for p in big)

@@ -114,8 +114,12 @@ def test_sweep_fp32_then_fp64_converges(tenet):
     non-monotone residual cannot loop."""
     from idmx.sweep import SweepPoint, run_sweep
     pts = [SweepPoint(L=64, G=-5.0 / 3, R=14.0), SweepPoint(L=48, G=-1.55, R=12.0)]
+    # march items while both domains live, tiles once the 64^2 one has
+    # converged (the end-of-sweep launch switch)
     res, fields = run_sweep(tenet, pts, mode='fp32+fp64', tol=1e-9, check_every=200,
-                            max_steps=40000)
+                            max_steps=40000, launch=(0, 128, 16, 0),
+                            launch_small=((1, 32, 8, 0), 64 * 64),
+                            launch32=(0, 64, 8, 0), launch32_small=((1, 16, 16, 0), 64 * 64))
     print('steps', res.steps, 'history tail', res.history[-3:], flush=True)
     assert res.converged.all() and not res.fallback.any()
     ref, fref = run_sweep(tenet, pts, mode='fp64', tol=1e-9, check_every=200, max_steps=40000)
