@@ -155,8 +155,11 @@ def test_every_launch_variant_matches_oracle(tenet, xi, es, dims):
     n0 = droplet_field(dims, seed=21)
     f_ref, _, _, _ = _oracle(xi, es, dims, n0, 12)
     eng = SCEngine(tenet, xi, es, dims, 'exp', G, 1.0)
-    for cand in tune.sim_candidates(len(dims), dims[-1]) + [
-            c[:2] + (3,) + c[3:] for c in tune.sim_candidates(len(dims), dims[-1])[:4]]:
+    cands = tune.sim_candidates(len(dims), dims[-1])
+    # the tuner's march lengths exceed these small domains, so also march
+    # short segments (segment boundaries, a last shorter segment)
+    for cand in cands + [c[:2] + (3,) + c[3:] for c in cands] + [
+            c[:2] + (4,) + c[3:] for c in cands[::5]]:
         print('variant', cand, flush=True)
         eng.set_launch(*cand)
         pop, swap, n, psi, u = _device_fields(tenet, eng, n0)
