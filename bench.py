@@ -32,6 +32,11 @@ import numpy as np  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 
+# fused-step variants (idmx_sc_set_launch)
+VARIANTS = {0: 'push, re-read', 1: 'push, registers', 2: 'push, re-read + prefetch',
+            3: 'push, re-read, >=3 waves/SIMD', 4: 'push, re-read + prefetch, >=3 waves/SIMD',
+            5: 'pull, registers', 6: 'pull, re-read'}
+
 WORKLOADS = {
     'd3q19_mrt_256': dict(xi='D3Q19', es='D3E4', dims=(256, 256, 256), coll='mrt',
                           prec='fp64', init='sphere3d'),
@@ -109,7 +114,7 @@ def run_gpu(name, steps, warmup, tune=True, launch=None):
         'B_alg': B_alg,
         'mass_rel_drift': abs(m1 - m0) / m0,
         'launch': {'tile_x': launch[0], 'tile_y': launch[1], 'planes_per_block': launch[2],
-                   'hold_in_registers': bool(launch[3])},
+                   'variant': launch[3], 'variant_name': VARIANTS.get(launch[3], '?')},
         'device': tenet.GetDeviceName(),
     }
     eng.close()

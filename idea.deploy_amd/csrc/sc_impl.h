@@ -29,7 +29,8 @@ struct Impl {
     return P;
   }
 
-  template <int TX, int TY, bool WF, bool HIST, bool PF = false, bool O3 = false>
+  template <int TX, int TY, bool WF, bool HIST, bool PF = false, bool O3 = false,
+            bool PULL = false>
   static hipError_t launch_fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
                                  double* psi, double* u) {
     const Geom& g = s->geo;
@@ -45,18 +46,23 @@ struct Impl {
                          dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H, ntx, nty,
                          ntiles, n, psi, u);
     else
-      hipLaunchKernelGGL((k_fused<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF>), dim3(grid),
-                         dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H, ntx, nty,
-                         ntiles, n, psi, u);
+      hipLaunchKernelGGL((k_fused<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF, PULL>),
+                         dim3(grid), dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H,
+                         ntx, nty, ntiles, n, psi, u);
     return hipGetLastError();
   }
 
   // variant: 1 populations held in registers, 0 re-read, 2 re-read with
   // prefetch of the next lead plane, 3 re-read with >= 3 waves/SIMD,
-  // 4 re-read with prefetch and >= 3 waves/SIMD
+  // 4 re-read with prefetch and >= 3 waves/SIMD; 5 and 6 are 1 and 0 on the
+  // pull form (run() converts to and from it around the steps)
   template <int TX, int TY>
   static hipError_t variant(idmx_sc_s* s, const T* pop, T* swap) {
     switch (s->hist) {
+      case 5: return launch_fused<TX, TY, false, true, false, false, true>(s, pop, swap, nullptr,
+                                                                          nullptr, nullptr);
+      case 6: return launch_fused<TX, TY, false, false, false, false, true>(s, pop, swap, nullptr,
+                                                                           nullptr, nullptr);
       case 0: return launch_fused<TX, TY, false, false>(s, pop, swap, nullptr, nullptr, nullptr);
       case 2: return launch_fused<TX, TY, false, false, true>(s, pop, swap, nullptr, nullptr,
                                                               nullptr);
@@ -74,6 +80,12 @@ struct Impl {
   static hipError_t fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
                           double* psi, double* u) {
     if constexpr (WF) {
+      if (s->hist >= 5) {
+        if constexpr (S::D == 2)
+          return launch_fused<64, 1, true, true, false, false, true>(s, pop, swap, n, psi, u);
+        else
+          return launch_fused<32, 8, true, true, false, false, true>(s, pop, swap, n, psi, u);
+      }
       if constexpr (S::D == 2) return launch_fused<64, 1, true, true>(s, pop, swap, n, psi, u);
       else return launch_fused<32, 8, true, true>(s, pop, swap, n, psi, u);
     } else {
@@ -122,6 +134,16 @@ struct Impl {
     }
     hipStream_t st = s->ctx->stream;
     IDMX_HIP(hipEventRecord(s->ctx->ev0, st));
+    // pull-form variants: convert the streamed populations to the pull form,
+    // step in it, convert back (nsteps + 2 buffer swaps: same parity as
+    // nsteps, so the caller's pointer bookkeeping is unchanged)
+    const bool pull = !twop && s->hist >= 5 && nsteps > 0;
+    if (pull) {
+      hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
+                         (const T*)*pop, (T*)*swap, s->geo, 1);
+      IDMX_HIP(hipGetLastError());
+      std::swap(*pop, *swap);
+    }
     for (int k = 0; k < nsteps; k++) {
       const bool last = k == nsteps - 1;
       const T* a = (const T*)*pop;
@@ -131,6 +153,12 @@ struct Impl {
       else if (wf && last) e = fused<true>(s, a, b, n, psi, u);
       else e = fused<false>(s, a, b, nullptr, nullptr, nullptr);
       if (e != hipSuccess) return hip_fail(e, "idmx_sc_run launch");
+      std::swap(*pop, *swap);
+    }
+    if (pull) {
+      hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
+                         (const T*)*pop, (T*)*swap, s->geo, -1);
+      IDMX_HIP(hipGetLastError());
       std::swap(*pop, *swap);
     }
     IDMX_HIP(hipEventRecord(s->ctx->ev1, st));

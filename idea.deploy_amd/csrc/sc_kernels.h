@@ -250,12 +250,67 @@ __device__ __forceinline__ void pop_st_nt(T* __restrict__ p, int q, int64_t V, i
 template <class S, class T>
 __device__ inline void push_store(T* __restrict__ dst, const Geom& geo, int x,
                                   int y, int z, const T (&fo)[S::Q]) {
+  // opaque site coordinates: the 19 target offsets are recomputed at every
+  // call instead of being hoisted out of the plane loop into ~2Q VGPRs
+  asm volatile("" : "+v"(x), "+v"(y));
 #pragma unroll
   for (int q = 0; q < S::Q; q++) {
     const int xt = S::cx[q] ? wrap(x + S::cx[q], geo.L0) : x;
     const int yt = S::cy[q] ? wrap(y + S::cy[q], geo.L1) : y;
     const int zt = S::cz[q] ? wrap(z + S::cz[q], geo.L2) : z;
     pop_st_nt(dst, q, geo.V, lex(geo, xt, yt, zt), fo[q]);
+  }
+}
+
+// Pull form of the populations: P_q(x) = f*_q(x) is the post-collision value
+// kept at its source site, so that f_q(x, t+1) = P_q(x - c_q). A step in
+// this form reads shifted (P_q at x - c_q: coalesced loads offset by one
+// element cost nothing measurable) and stores every population at its own
+// site (aligned), where the push form stores shifted, which costs ~17 % of
+// a 19-stream copy. pull_src is the source of population q of the site at
+// in-plane position (sx, sy) of marching plane mj.
+template <class S>
+__device__ __forceinline__ int64_t pull_src(const Geom& geo, int q, int sx, int sy, int mj) {
+  if constexpr (S::D == 3) {
+    const int xs = S::cx[q] ? wrap(sx - S::cx[q], geo.L0) : sx;
+    const int ys = S::cy[q] ? wrap(sy - S::cy[q], geo.L1) : sy;
+    const int zs = S::cz[q] ? wrap(mj - S::cz[q], geo.L2) : mj;
+    return lex(geo, xs, ys, zs);
+  } else {
+    const int xs = S::cx[q] ? wrap(sx - S::cx[q], geo.L0) : sx;
+    const int ys = S::cy[q] ? wrap(mj - S::cy[q], geo.L1) : mj;
+    return (int64_t)xs + (int64_t)geo.L0 * ys;
+  }
+}
+
+// Store one site's f* at the site itself (pull form).
+template <class S, class T>
+__device__ inline void local_store(T* __restrict__ dst, const Geom& geo, int x, int y, int z,
+                                   const T (&fo)[S::Q]) {
+  const int64_t g = lex(geo, x, y, z);
+#pragma unroll
+  for (int q = 0; q < S::Q; q++) pop_st_nt(dst, q, geo.V, g, fo[q]);
+}
+
+// Conversion between streamed populations and the pull form:
+// dir = +1: dst_q(x) = src_q(x + c_q) (streamed -> pull form),
+// dir = -1: dst_q(x) = src_q(x - c_q) (pull form -> streamed).
+template <class S, class T>
+__global__ void __launch_bounds__(256)
+k_restream(const T* __restrict__ src, T* __restrict__ dst, Geom geo, int dir) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t L01 = (int64_t)geo.L0 * geo.L1;
+    const int z = (int)(g / L01);
+    const int64_t r = g - (int64_t)z * L01;
+    const int y = (int)(r / geo.L0), x = (int)(r - (int64_t)y * geo.L0);
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) {
+      const int xs = S::cx[q] ? wrap(x + dir * S::cx[q], geo.L0) : x;
+      const int ys = S::cy[q] ? wrap(y + dir * S::cy[q], geo.L1) : y;
+      const int zs = S::cz[q] ? wrap(z + dir * S::cz[q], geo.L2) : z;
+      dst[(size_t)q * geo.V + g] = src[(size_t)q * geo.V + lex(geo, xs, ys, zs)];
+    }
   }
 }
 
@@ -287,8 +342,11 @@ __device__ inline void push_store(T* __restrict__ dst, const Geom& geo, int x,
 // PF (with HIST = false) issues the loads of the next lead plane right after
 // the barrier, before the collision, so their latency overlaps the collision
 // arithmetic and stores.
+//
+// PULL runs the step on the pull form (pull_src above): every population
+// read is taken from its source site, and f* is stored at the site itself.
 template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY,
-          bool HIST, bool PF>
+          bool HIST, bool PF, bool PULL = false>
 __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restrict__ pop_swap,
                                            const Geom& geo, const Phys<T>& P, int H, int ntx,
                                            int nty, int ntiles, double* __restrict__ n_out,
@@ -356,17 +414,40 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
     hys[k] = hy;
     const int gx = ((x0 - R + hx) % geo.L0 + geo.L0) % geo.L0;
     const int gy = D == 3 ? ((y0 - R + hy) % geo.L1 + geo.L1) % geo.L1 : 0;
-    hoff[k] = (int64_t)gx + (int64_t)geo.L0 * gy;
+    hoff[k] = PULL ? (int64_t)gx + ((int64_t)gy << 32)  // (x, y) of the halo site
+                   : (int64_t)gx + (int64_t)geo.L0 * gy;
   }
 
   constexpr int HR = HIST ? R + 1 : 1;
   T fh[HR][Q];  // own populations of the last R+1 lead planes (HIST)
   T dnh[HR];    // their sums
   T fn[PF ? Q : 1];  // prefetched own populations of the next lead plane
+  // population q of this thread's own site in marching plane mj. In the
+  // pull form the source offset is rebuilt from the site's in-plane offset
+  // and its four edge flags for every plane (a few adds and selects), so
+  // that the compiler does not keep 19 hoisted offsets live in VGPRs.
+  const bool ex0 = xw == 0, exL = xw == geo.L0 - 1;
+  const bool ey0 = D == 3 && yw == 0, eyL = D == 3 && yw == geo.L1 - 1;
+  auto own_ld = [&](int q, int mj) -> T {
+    if constexpr (PULL) {
+      int64_t o = own_off;
+      asm volatile("" : "+v"(o));  // opaque: recomputed per plane, not hoisted
+      o -= S::cx[q] + (D == 3 ? (int64_t)geo.L0 * S::cy[q] : 0);
+      if (S::cx[q] > 0 && ex0) o += geo.L0;
+      if (S::cx[q] < 0 && exL) o -= geo.L0;
+      if (D == 3 && S::cy[q] > 0 && ey0) o += PS;
+      if (D == 3 && S::cy[q] < 0 && eyL) o -= PS;
+      const int cm = D == 3 ? S::cz[q] : S::cy[q];
+      const int ms = cm ? wrap(mj - cm, Lm) : mj;
+      return pop_ld(pop, q, geo.V, (int64_t)ms * PS + o);
+    } else {
+      return pop_ld(pop, q, geo.V, (int64_t)mj * PS + own_off);
+    }
+  };
   if (PF) {
-    const int64_t b0 = (int64_t)(((m0 - R) % Lm + Lm) % Lm) * PS;
+    const int mp = ((m0 - R) % Lm + Lm) % Lm;
 #pragma unroll
-    for (int q = 0; q < Q; q++) fn[q] = pop_ld(pop, q, geo.V, b0 + own_off);
+    for (int q = 0; q < Q; q++) fn[q] = own_ld(q, mp);
   }
 
   for (int j = 0; j < Hs + 2 * R; j++) {
@@ -380,7 +461,7 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
         for (int q = 0; q < Q; q++) fh[HR - 1][q] = fn[q];
       } else {
 #pragma unroll
-        for (int q = 0; q < Q; q++) fh[HR - 1][q] = pop_ld(pop, q, geo.V, base + own_off);
+        for (int q = 0; q < Q; q++) fh[HR - 1][q] = own_ld(q, mj);
       }
 #pragma unroll
       for (int q = 0; q < Q; q++) s += fh[HR - 1][q];
@@ -392,24 +473,32 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
       if (tid + k * NT < NH) {
         T s = 0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) s += pop_ld(pop, q, geo.V, base + hoff[k]);
+        for (int q = 0; q < Q; q++) {
+          if constexpr (PULL) {
+            int64_t hp = hoff[k];
+            asm volatile("" : "+v"(hp));  // opaque: recomputed per plane, not hoisted
+            s += pop_ld(pop, q, geo.V,
+                        pull_src<S>(geo, q, (int)(hp & 0xffffffff), (int)(hp >> 32), mj));
+          }
+          else
+            s += pop_ld(pop, q, geo.V, base + hoff[k]);
+        }
         lds[slot][hys[k]][hxs[k]] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
       }
     }
     __syncthreads();
     if (PF && j + 1 < Hs + 2 * R) {
-      const int64_t bn = (int64_t)(((m0 - R + j + 1) % Lm + Lm) % Lm) * PS;
+      const int mn = ((m0 - R + j + 1) % Lm + Lm) % Lm;
 #pragma unroll
-      for (int q = 0; q < Q; q++) fn[q] = pop_ld(pop, q, geo.V, bn + own_off);
+      for (int q = 0; q < Q; q++) fn[q] = own_ld(q, mn);
     }
     if (j >= 2 * R && own_valid) {
       // plane with lead index j - R
       const int mc = m0 + j - 2 * R;
       if (!HIST) {
-        const int64_t cbase = (int64_t)mc * PS;
         T s = 0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) fh[0][q] = pop_ld(pop, q, geo.V, cbase + own_off);
+        for (int q = 0; q < Q; q++) fh[0][q] = own_ld(q, mc);
 #pragma unroll
         for (int q = 0; q < Q; q++) s += fh[0][q];
         dnh[0] = s;
@@ -434,7 +523,8 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
       T fo[Q], u[3];
       collide<S, T, SHIFT, COLL>(fh[0], dnh[0], gx, gy, gz, lpsi, P, fo, u);
       const int xc = x, yc = D == 3 ? y : mc, zc = D == 3 ? mc : 0;
-      push_store<S, T>(pop_swap, geo, xc, yc, zc, fo);
+      if constexpr (PULL) local_store<S, T>(pop_swap, geo, xc, yc, zc, fo);
+      else push_store<S, T>(pop_swap, geo, xc, yc, zc, fo);
       if (WF) {
         const int64_t g = lex(geo, xc, yc, zc);
         const T dn = dnh[0];
@@ -457,13 +547,13 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
 }
 
 template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY,
-          bool HIST = true, bool PF = false>
+          bool HIST = true, bool PF = false, bool PULL = false>
 __global__ void __launch_bounds__(TX * TY)
 k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<T> P, int H,
         int ntx, int nty, int ntiles, double* __restrict__ n_out, double* __restrict__ psi_out,
         double* __restrict__ u_out) {
-  fused_body<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF>(pop, pop_swap, geo, P, H, ntx, nty,
-                                                          ntiles, n_out, psi_out, u_out);
+  fused_body<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF, PULL>(
+      pop, pop_swap, geo, P, H, ntx, nty, ntiles, n_out, psi_out, u_out);
 }
 
 // Same kernel compiled for at least 3 waves per SIMD (the register budget

@@ -35,7 +35,7 @@ def _store(key, val):
 def sim_candidates(D, L_march):
     tiles = ((64, 1), (128, 1), (256, 1)) if D == 2 else ((32, 8), (64, 4), (128, 2), (256, 2))
     Hs = [h for h in (32, 64, 128) if h <= L_march] or [L_march]
-    return [(tx, ty, h, var) for tx, ty in tiles for h in Hs for var in (1, 0, 2, 3, 4)]
+    return [(tx, ty, h, var) for tx, ty in tiles for h in Hs for var in (1, 0, 2, 3, 4, 5, 6)]
 
 
 def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
@@ -67,8 +67,11 @@ def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
     for cand in sim_candidates(D, engine.dims[-1]):
         engine.set_launch(*cand)
         engine.run(a, b, 2)
-        engine.run(a, b, steps)
-        timings[','.join(map(str, cand))] = engine.last_run_ms() / steps
+        t_short = engine.last_run_ms()
+        engine.run(a, b, 2 + steps)
+        # difference of two run lengths: the per-run costs (the pull form's
+        # two conversion passes) cancel, so every variant is timed per step
+        timings[','.join(map(str, cand))] = max(engine.last_run_ms() - t_short, 1e-6) / steps
     best = min(timings, key=timings.get)
     best = tuple(int(v) for v in best.split(','))
     engine.set_launch(*(best if apply else old))

@@ -207,7 +207,11 @@ int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
  * per block, and the variant `hist`: 1 holds the tile's populations in
  * registers between their read and their collision, 0 re-reads them, 2
  * re-reads them and prefetches the next plane before the collision, 3 and 4
- * are 0 and 2 compiled for at least 3 waves per SIMD. Chosen at creation;
+ * are 0 and 2 compiled for at least 3 waves per SIMD, 5 and 6 are 1 and 0
+ * run on the pull form (post-collision values stored at their own site,
+ * read from the source site; idmx_sc_run converts to that form before the
+ * steps and back after them, so pop always holds streamed populations
+ * between calls). Chosen at creation;
  * the autotuner (idmx.tune) overrides it. No reference counterpart: the
  * reference fixes block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
 int idmx_sc_set_launch(idmx_sc_t sim, int tx, int ty, int H, int hist);
