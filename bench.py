@@ -35,7 +35,8 @@ PEAK_HBM_GBS = 8000.0
 # fused-step variants (idmx_sc_set_launch)
 VARIANTS = {0: 'push, re-read', 1: 'push, registers', 2: 'push, re-read + prefetch',
             3: 'push, re-read, >=3 waves/SIMD', 4: 'push, re-read + prefetch, >=3 waves/SIMD',
-            5: 'pull, registers', 6: 'pull, re-read'}
+            5: 'pull, registers', 6: 'pull, re-read', 7: 'push, re-read, 2 planes/barrier',
+            8: 'pull, re-read, 2 planes/barrier'}
 
 WORKLOADS = {
     'd3q19_mrt_256': dict(xi='D3Q19', es='D3E4', dims=(256, 256, 256), coll='mrt',

@@ -453,7 +453,7 @@ int idmx_sc_set_launch(idmx_sc_t s, int tx, int ty, int H, int hist) {
   s->tx = tx;
   s->ty = ty;
   s->H = H;
-  CHECK_ARG(hist >= 0 && hist <= 6, "idmx_sc_set_launch: variant is 0..6");
+  CHECK_ARG(hist >= 0 && hist <= 8, "idmx_sc_set_launch: variant is 0..8");
   s->hist = hist;
   return IDMX_OK;
 }

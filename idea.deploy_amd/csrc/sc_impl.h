@@ -13,6 +13,9 @@ inline unsigned grid_for(int64_t V) {
   return (unsigned)std::min<int64_t>(b, 65536);
 }
 
+// Launch variants that run on the pull form of the populations.
+inline bool pull_form(int variant) { return variant == 5 || variant == 6 || variant == 8; }
+
 template <class S, class ES, class T, bool SHIFT, int COLL>
 struct Impl {
   static Phys<T> phys(const idmx_sc_s* s) {
@@ -30,7 +33,7 @@ struct Impl {
   }
 
   template <int TX, int TY, bool WF, bool HIST, bool PF = false, bool O3 = false,
-            bool PULL = false>
+            bool PULL = false, bool P2 = false>
   static hipError_t launch_fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
                                  double* psi, double* u) {
     const Geom& g = s->geo;
@@ -46,7 +49,7 @@ struct Impl {
                          dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H, ntx, nty,
                          ntiles, n, psi, u);
     else
-      hipLaunchKernelGGL((k_fused<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF, PULL>),
+      hipLaunchKernelGGL((k_fused<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF, PULL, P2>),
                          dim3(grid), dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H,
                          ntx, nty, ntiles, n, psi, u);
     return hipGetLastError();
@@ -55,10 +58,15 @@ struct Impl {
   // variant: 1 populations held in registers, 0 re-read, 2 re-read with
   // prefetch of the next lead plane, 3 re-read with >= 3 waves/SIMD,
   // 4 re-read with prefetch and >= 3 waves/SIMD; 5 and 6 are 1 and 0 on the
-  // pull form (run() converts to and from it around the steps)
+  // pull form (run() converts to and from it around the steps); 7 and 8
+  // are 0 and 6 with two lead planes per barrier
   template <int TX, int TY>
   static hipError_t variant(idmx_sc_s* s, const T* pop, T* swap) {
     switch (s->hist) {
+      case 7: return launch_fused<TX, TY, false, false, false, false, false, true>(
+          s, pop, swap, nullptr, nullptr, nullptr);
+      case 8: return launch_fused<TX, TY, false, false, false, false, true, true>(
+          s, pop, swap, nullptr, nullptr, nullptr);
       case 5: return launch_fused<TX, TY, false, true, false, false, true>(s, pop, swap, nullptr,
                                                                           nullptr, nullptr);
       case 6: return launch_fused<TX, TY, false, false, false, false, true>(s, pop, swap, nullptr,
@@ -80,7 +88,7 @@ struct Impl {
   static hipError_t fused(idmx_sc_s* s, const T* pop, T* swap, double* n,
                           double* psi, double* u) {
     if constexpr (WF) {
-      if (s->hist >= 5) {
+      if (pull_form(s->hist)) {
         if constexpr (S::D == 2)
           return launch_fused<64, 1, true, true, false, false, true>(s, pop, swap, n, psi, u);
         else
@@ -137,7 +145,7 @@ struct Impl {
     // pull-form variants: convert the streamed populations to the pull form,
     // step in it, convert back (nsteps + 2 buffer swaps: same parity as
     // nsteps, so the caller's pointer bookkeeping is unchanged)
-    const bool pull = !twop && s->hist >= 5 && nsteps > 0;
+    const bool pull = !twop && pull_form(s->hist) && nsteps > 0;
     if (pull) {
       hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
                          (const T*)*pop, (T*)*swap, s->geo, 1);

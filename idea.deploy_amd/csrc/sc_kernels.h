@@ -346,18 +346,21 @@ k_restream(const T* __restrict__ src, T* __restrict__ dst, Geom geo, int dir) {
 // PULL runs the step on the pull form (pull_src above): every population
 // read is taken from its source site, and f* is stored at the site itself.
 template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY,
-          bool HIST, bool PF, bool PULL = false>
+          bool HIST, bool PF, bool PULL = false, bool P2 = false>
 __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restrict__ pop_swap,
                                            const Geom& geo, const Phys<T>& P, int H, int ntx,
                                            int nty, int ntiles, double* __restrict__ n_out,
                                            double* __restrict__ psi_out,
                                            double* __restrict__ u_out) {
   static_assert(!(PF && HIST), "prefetch is for the re-read variant");
+  static_assert(!(P2 && (HIST || PF)), "two planes per barrier is a re-read variant");
   constexpr int D = S::D, Q = S::Q, R = ES::R;
   constexpr int NT = TX * TY;
   constexpr int PX = TX + 2 * R;
   constexpr int PY = D == 3 ? TY + 2 * R : 1;
-  constexpr int NS = 2 * R + 2;
+  // ring of psi planes: one slot more than the stencil reach needs (two
+  // more with P2, which writes two planes before each barrier)
+  constexpr int NS = P2 ? 2 * R + 4 : 2 * R + 2;
   constexpr int NH = PX * PY - NT;  // halo sites per plane
   static_assert(D == 3 || TY == 1, "2D tiles are rows");
   constexpr int NHT = (NH + NT - 1) / NT;  // halo sites per thread
@@ -448,6 +451,83 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
     const int mp = ((m0 - R) % Lm + Lm) % Lm;
 #pragma unroll
     for (int q = 0; q < Q; q++) fn[q] = own_ld(q, mp);
+  }
+
+  if constexpr (P2) {
+    // Two lead planes per barrier: the loads of both planes are in flight
+    // together and the per-plane barrier cost is halved. Same arithmetic.
+    const int J = Hs + 2 * R;
+    auto lead = [&](int j) {
+      const int mj = ((m0 - R + j) % Lm + Lm) % Lm;
+      const int64_t base = (int64_t)mj * PS;
+      const int slot = j % NS;
+      T s = 0;
+#pragma unroll
+      for (int q = 0; q < Q; q++) s += own_ld(q, mj);
+      lds[slot][D == 3 ? R + iy : 0][R + ix] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+#pragma unroll
+      for (int k = 0; k < NHT; k++) {
+        if (tid + k * NT < NH) {
+          T sh = 0;
+#pragma unroll
+          for (int q = 0; q < Q; q++) {
+            if constexpr (PULL) {
+              int64_t hp = hoff[k];
+              asm volatile("" : "+v"(hp));
+              sh += pop_ld(pop, q, geo.V,
+                           pull_src<S>(geo, q, (int)(hp & 0xffffffff), (int)(hp >> 32), mj));
+            } else {
+              sh += pop_ld(pop, q, geo.V, base + hoff[k]);
+            }
+          }
+          lds[slot][hys[k]][hxs[k]] = psi_of(P.psi_kind, SHIFT ? P.rho0 + sh : sh);
+        }
+      }
+    };
+    auto coll = [&](int j) {
+      const int mc = m0 + j - 2 * R;
+      T f[Q];
+      T s = 0;
+#pragma unroll
+      for (int q = 0; q < Q; q++) f[q] = own_ld(q, mc);
+#pragma unroll
+      for (int q = 0; q < Q; q++) s += f[q];
+      const int cy = D == 3 ? R + iy : 0;
+      const int cx = R + ix;
+      T gx = 0, gy = 0, gz = 0;
+#pragma unroll
+      for (int e = 0; e < ES::QE; e++) {
+        const int em = D == 3 ? ES::ez[e] : ES::ey[e];
+        const int ey = D == 3 ? ES::ey[e] : 0;
+        const T v = lds[(j - R + em) % NS][cy + ey][cx + ES::ex[e]];
+        const T w = T(ES::w[e]);
+        if (ES::ex[e] != 0) gx += T(ES::ex[e]) * w * v;
+        if (ES::ey[e] != 0) gy += T(ES::ey[e]) * w * v;
+        if (ES::ez[e] != 0) gz += T(ES::ez[e]) * w * v;
+      }
+      const T lpsi = lds[(j - R) % NS][cy][cx];
+      T fo[Q], u[3];
+      collide<S, T, SHIFT, COLL>(f, s, gx, gy, gz, lpsi, P, fo, u);
+      const int xc = x, yc = D == 3 ? y : mc, zc = D == 3 ? mc : 0;
+      if constexpr (PULL) local_store<S, T>(pop_swap, geo, xc, yc, zc, fo);
+      else push_store<S, T>(pop_swap, geo, xc, yc, zc, fo);
+      if (WF) {
+        const int64_t g = lex(geo, xc, yc, zc);
+        n_out[g] = (double)(SHIFT ? P.rho0 + s : s);
+        psi_out[g] = (double)lpsi;
+        u_out[g] = (double)u[0];
+        u_out[g + geo.V] = (double)u[1];
+        if (D == 3) u_out[g + 2 * geo.V] = (double)u[2];
+      }
+    };
+    for (int j = 0; j < J; j += 2) {
+      lead(j);
+      if (j + 1 < J) lead(j + 1);
+      __syncthreads();
+      if (own_valid && j >= 2 * R) coll(j);
+      if (own_valid && j + 1 < J && j + 1 >= 2 * R) coll(j + 1);
+    }
+    return;
   }
 
   for (int j = 0; j < Hs + 2 * R; j++) {
@@ -547,12 +627,12 @@ __device__ __forceinline__ void fused_body(const T* __restrict__ pop, T* __restr
 }
 
 template <class S, class ES, class T, bool SHIFT, int COLL, bool WF, int TX, int TY,
-          bool HIST = true, bool PF = false, bool PULL = false>
+          bool HIST = true, bool PF = false, bool PULL = false, bool P2 = false>
 __global__ void __launch_bounds__(TX * TY)
 k_fused(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<T> P, int H,
         int ntx, int nty, int ntiles, double* __restrict__ n_out, double* __restrict__ psi_out,
         double* __restrict__ u_out) {
-  fused_body<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF, PULL>(
+  fused_body<S, ES, T, SHIFT, COLL, WF, TX, TY, HIST, PF, PULL, P2>(
       pop, pop_swap, geo, P, H, ntx, nty, ntiles, n_out, psi_out, u_out);
 }
 

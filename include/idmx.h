@@ -211,7 +211,8 @@ int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
  * run on the pull form (post-collision values stored at their own site,
  * read from the source site; idmx_sc_run converts to that form before the
  * steps and back after them, so pop always holds streamed populations
- * between calls). Chosen at creation;
+ * between calls), 7 and 8 are 0 and 6 with two lead planes loaded per
+ * barrier. Chosen at creation;
  * the autotuner (idmx.tune) overrides it. No reference counterpart: the
  * reference fixes block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
 int idmx_sc_set_launch(idmx_sc_t sim, int tx, int ty, int H, int hist);
