@@ -212,12 +212,16 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     t0 = time.perf_counter()
     starts = warmstart.predict_fields(tenet, net, eval_pts)
     t_inf = time.perf_counter() - t0
-    runs = {'batched_cold_fp64': (None, 'fp64'), 'batched_cold': (None, 'fp32+fp64'),
-            'batched_warm_fp64': (starts, 'fp64'), 'batched_warm': (starts, 'fp32+fp64')}
+    # the last run keeps every learned start (no fallback on a growing
+    # residual, only on a non-finite one): what the fallbacks cost or save
+    runs = {'batched_cold_fp64': (None, 'fp64', 5), 'batched_cold': (None, 'fp32+fp64', 5),
+            'batched_warm_fp64': (starts, 'fp64', 5), 'batched_warm': (starts, 'fp32+fp64', 5),
+            'batched_warm_fp64_no_fallback': (starts, 'fp64', 10 ** 9)}
     fields = {}
-    for key, (st, mode) in runs.items():
+    for key, (st, mode, gc) in runs.items():
         res, f = sweep.run_sweep(tenet, eval_pts, start=st, mode=mode, launch=l64,
-                                 launch32=l32, launch_small=s64, launch32_small=s32)
+                                 launch32=l32, launch_small=s64, launch32_small=s32,
+                                 grow_checks=gc)
         fields[key] = f
         wall = res.wall_s + (t_inf if st is not None else 0.0)
         out[key] = {'wall_s': wall, 'gpu_ms': res.gpu_ms, 'steps_mean': float(res.steps.mean()),

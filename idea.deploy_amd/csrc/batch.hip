@@ -14,7 +14,8 @@ using namespace idmx;
 struct idmx_batch_s {
   idmx_ctx_s* ctx = nullptr;
   int prec = 0, estencil = 4;
-  int kind = 1;         // 0: marching items (tx x H rows), 1: tiles (tx x H sites)
+  int kind = 1;  // 0: marching items (tx x H rows), 1: tiles (tx x H sites), 2/3: 0/1 on the
+                 // pull form (converted before and after each run)
   int tx = 32, H = 8;
   int ndom = 0;
   std::vector<BatchDom> doms;
@@ -39,17 +40,44 @@ struct idmx_batch_s {
 
 namespace {
 
+template <class ES, class T, bool SHIFT, int TX, bool PULL>
+hipError_t launch_march(idmx_batch_s* b, const void* in, void* out) {
+  hipLaunchKernelGGL((k_batched<ES, T, SHIFT, TX, PULL>), dim3(b->grid), dim3(TX), 0,
+                     b->ctx->stream, (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems);
+  return hipGetLastError();
+}
+
 template <class ES, class T, bool SHIFT, int TX>
 hipError_t launch_step(idmx_batch_s* b, const void* in, void* out) {
-  hipLaunchKernelGGL((k_batched<ES, T, SHIFT, TX>), dim3(b->grid), dim3(TX), 0, b->ctx->stream,
-                     (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems);
-  return hipGetLastError();
+  return b->kind == 2 ? launch_march<ES, T, SHIFT, TX, true>(b, in, out)
+                      : launch_march<ES, T, SHIFT, TX, false>(b, in, out);
 }
 
 template <class ES, class T, bool SHIFT, int BX, int BY>
 hipError_t launch_tile(idmx_batch_s* b, const void* in, void* out) {
-  hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY>), dim3(b->grid), dim3(BX * BY), 0,
-                     b->ctx->stream, (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems);
+  if (b->kind == 3)
+    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, true>), dim3(b->grid),
+                       dim3(BX * BY), 0, b->ctx->stream, (const T*)in, (T*)out, b->d_doms,
+                       b->d_items, b->d_nitems);
+  else
+    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, false>), dim3(b->grid),
+                       dim3(BX * BY), 0, b->ctx->stream, (const T*)in, (T*)out, b->d_doms,
+                       b->d_items, b->d_nitems);
+  return hipGetLastError();
+}
+
+// Streamed <-> pull form of every live domain (kinds 2 and 3).
+hipError_t restream(idmx_batch_s* b, int dir) {
+  const void* in = b->pop[b->cur];
+  void* out = b->pop[b->cur ^ 1];
+  if (b->prec == IDMX_PREC_F64)
+    hipLaunchKernelGGL(k_batched_restream<double>, dim3(b->grid), dim3(256), 0, b->ctx->stream,
+                       (const double*)in, (double*)out, b->d_doms, b->d_items, b->d_nitems,
+                       b->tx, dir);
+  else
+    hipLaunchKernelGGL(k_batched_restream<float>, dim3(b->grid), dim3(256), 0, b->ctx->stream,
+                       (const float*)in, (float*)out, b->d_doms, b->d_items, b->d_nitems, b->tx,
+                       dir);
   return hipGetLastError();
 }
 
@@ -61,7 +89,7 @@ bool tile_shape_ok(int tx, int H) {
 
 template <class ES, class T, bool SHIFT>
 hipError_t step_tx(idmx_batch_s* b, const void* in, void* out) {
-  if (b->kind == 1) {
+  if (b->kind == 1 || b->kind == 3) {
     if (b->tx == 64) return launch_tile<ES, T, SHIFT, 64, 4>(b, in, out);
     if (b->tx == 16) return launch_tile<ES, T, SHIFT, 16, 16>(b, in, out);
     if (b->H == 4) return launch_tile<ES, T, SHIFT, 32, 4>(b, in, out);
@@ -224,8 +252,9 @@ int idmx_batch_destroy(idmx_batch_t b) {
 
 int idmx_batch_set_launch(idmx_batch_t b, int kind, int tx, int H, int grid) {
   BCHECK(b, "idmx_batch_set_launch: null batch");
-  BCHECK(kind == 0 || kind == 1, "idmx_batch_set_launch: kind is 0 (march) or 1 (tile)");
-  if (kind == 0) {
+  BCHECK(kind >= 0 && kind <= 3,
+         "idmx_batch_set_launch: kind is 0 (march), 1 (tile), 2 / 3 (pull-form march / tile)");
+  if (kind == 0 || kind == 2) {
     BCHECK(tx == 64 || tx == 128 || tx == 256, "idmx_batch_set_launch: march tx is 64, 128 or 256");
     BCHECK(H >= 1, "idmx_batch_set_launch: H >= 1");
   } else {
@@ -302,6 +331,14 @@ int idmx_batch_run(idmx_batch_t b, int nsteps) {
   BCHECK(b && nsteps >= 0, "idmx_batch_run: invalid argument");
   hipStream_t st = b->ctx->stream;
   IDMX_HIP(hipEventRecord(b->ctx->ev0, st));
+  const bool pull = (b->kind == 2 || b->kind == 3) && b->n_items > 0 && nsteps > 0;
+  if (pull) {
+    // to the pull form; the buffer flips, so a graph captured after this
+    // point always sees the same parity (the conversion back flips it again)
+    hipError_t e = restream(b, 1);
+    if (e != hipSuccess) return hip_fail(e, "idmx_batch_run restream");
+    b->cur ^= 1;
+  }
   if (b->n_items > 0 && nsteps > 0) {
     // graph of an even number of steps (A->B, B->A, ...): replays keep the
     // state in pop[cur]
@@ -336,6 +373,11 @@ int idmx_batch_run(idmx_batch_t b, int nsteps) {
       b->cur ^= 1;
       drop_graph(b);  // parity changed: the captured buffers no longer match
     }
+  }
+  if (pull) {
+    hipError_t e = restream(b, -1);
+    if (e != hipSuccess) return hip_fail(e, "idmx_batch_run restream");
+    b->cur ^= 1;
   }
   IDMX_HIP(hipEventRecord(b->ctx->ev1, st));
   IDMX_HIP(hipEventSynchronize(b->ctx->ev1));

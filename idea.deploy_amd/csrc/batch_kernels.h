@@ -33,7 +33,7 @@ struct BatchItem {
 // One TX-wide, Hs-row march of the 2D fused step on one domain (R = stencil
 // reach). Same structure as k_fused's 2D path; see there for the ring and
 // register-history logic.
-template <class ES, class T, bool SHIFT, int TX>
+template <class ES, class T, bool SHIFT, int TX, bool PULL = false>
 __device__ inline void march2d(const T* __restrict__ pop, T* __restrict__ pop_swap,
                                const Geom& geo, const Phys<T>& P, int x0, int m0,
                                int Hs, T (*lds)[TX + 2 * ES::R]) {
@@ -59,7 +59,9 @@ __device__ inline void march2d(const T* __restrict__ pop, T* __restrict__ pop_sw
     {
       T s = 0;
 #pragma unroll
-      for (int q = 0; q < Q; q++) fh[R][q] = pop[(size_t)q * geo.V + base + own_off];
+      for (int q = 0; q < Q; q++)
+        fh[R][q] = pop[(size_t)q * geo.V +
+                       (PULL ? pull_src<S>(geo, q, (int)own_off, 0, mj) : base + own_off)];
 #pragma unroll
       for (int q = 0; q < Q; q++) s += fh[R][q];
       dnh[R] = s;
@@ -68,7 +70,9 @@ __device__ inline void march2d(const T* __restrict__ pop, T* __restrict__ pop_sw
     if (has_halo) {
       T s = 0;
 #pragma unroll
-      for (int q = 0; q < Q; q++) s += pop[(size_t)q * geo.V + base + halo_off];
+      for (int q = 0; q < Q; q++)
+        s += pop[(size_t)q * geo.V +
+                 (PULL ? pull_src<S>(geo, q, (int)halo_off, 0, mj) : base + halo_off)];
       lds[slot][hx] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
     }
     __syncthreads();
@@ -88,7 +92,8 @@ __device__ inline void march2d(const T* __restrict__ pop, T* __restrict__ pop_sw
       const T lpsi = lds[sl[R]][cx];
       T fo[Q], u[3];
       collide<S, T, SHIFT, 0>(fh[0], dnh[0], gx, gy, gz, lpsi, P, fo, u);
-      push_store<S, T>(pop_swap, geo, x, m0 + j - 2 * R, 0, fo);
+      if constexpr (PULL) local_store<S, T>(pop_swap, geo, x, m0 + j - 2 * R, 0, fo);
+      else push_store<S, T>(pop_swap, geo, x, m0 + j - 2 * R, 0, fo);
     }
 #pragma unroll
     for (int k = 0; k < R; k++) {
@@ -110,7 +115,7 @@ __device__ inline Phys<T> dom_phys(const BatchDom& d) {
   return P;
 }
 
-template <class ES, class T, bool SHIFT, int TX>
+template <class ES, class T, bool SHIFT, int TX, bool PULL = false>
 __global__ void __launch_bounds__(TX)
 k_batched(const T* __restrict__ pop_all, T* __restrict__ swap_all,
           const BatchDom* __restrict__ doms, const BatchItem* __restrict__ items,
@@ -122,8 +127,8 @@ k_batched(const T* __restrict__ pop_all, T* __restrict__ swap_all,
     const BatchDom d = doms[w.dom];
     const Geom geo{d.L0, d.L1, 1, (int64_t)d.L0 * d.L1};
     const Phys<T> P = dom_phys<T>(d);
-    march2d<ES, T, SHIFT, TX>(pop_all + d.pop_off, swap_all + d.pop_off, geo, P, w.x0,
-                              w.m0, w.Hs, lds);
+    march2d<ES, T, SHIFT, TX, PULL>(pop_all + d.pop_off, swap_all + d.pop_off, geo, P, w.x0,
+                                    w.m0, w.Hs, lds);
     __syncthreads();
   }
 }
@@ -148,7 +153,7 @@ __device__ inline int pmod(int v, int L) {
 // what is left of the domain) still loads the periodic image of its
 // position: that is the halo its valid neighbours need. Only sites inside
 // the domain collide and store.
-template <class ES, class T, bool SHIFT, int BX, int BY>
+template <class ES, class T, bool SHIFT, int BX, int BY, bool PULL = false>
 __global__ void __launch_bounds__(BX * BY)
 k_batched_tile(const T* __restrict__ pop_all, T* __restrict__ swap_all,
                const BatchDom* __restrict__ doms, const BatchItem* __restrict__ items,
@@ -170,9 +175,11 @@ k_batched_tile(const T* __restrict__ pop_all, T* __restrict__ swap_all,
     T f[Q];
     T s = 0;
     {
-      const int64_t g = (int64_t)pmod(y, d.L1) * d.L0 + pmod(x, d.L0);
+      const int xs = pmod(x, d.L0), ys = pmod(y, d.L1);
+      const int64_t g = (int64_t)ys * d.L0 + xs;
 #pragma unroll
-      for (int q = 0; q < Q; q++) f[q] = pop[(size_t)q * geo.V + g];
+      for (int q = 0; q < Q; q++)
+        f[q] = pop[(size_t)q * geo.V + (PULL ? pull_src<S>(geo, q, xs, 0, ys) : g)];
 #pragma unroll
       for (int q = 0; q < Q; q++) s += f[q];
       lds[ty + R][tx + R] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
@@ -189,11 +196,12 @@ k_batched_tile(const T* __restrict__ pop_all, T* __restrict__ swap_all,
         hy = R + r;
         hx = c < R ? c : BX + c;
       }
-      const int64_t g =
-          (int64_t)pmod(w.m0 - R + hy, d.L1) * d.L0 + pmod(w.x0 - R + hx, d.L0);
+      const int hxs = pmod(w.x0 - R + hx, d.L0), hys = pmod(w.m0 - R + hy, d.L1);
+      const int64_t g = (int64_t)hys * d.L0 + hxs;
       T sh = 0;
 #pragma unroll
-      for (int q = 0; q < Q; q++) sh += pop[(size_t)q * geo.V + g];
+      for (int q = 0; q < Q; q++)
+        sh += pop[(size_t)q * geo.V + (PULL ? pull_src<S>(geo, q, hxs, 0, hys) : g)];
       lds[hy][hx] = psi_of(P.psi_kind, SHIFT ? P.rho0 + sh : sh);
     }
     __syncthreads();
@@ -208,7 +216,8 @@ k_batched_tile(const T* __restrict__ pop_all, T* __restrict__ swap_all,
       }
       T fo[Q], u[3];
       collide<S, T, SHIFT, 0>(f, s, gx, gy, T(0), lds[ty + R][tx + R], P, fo, u);
-      push_store<S, T>(swap_all + d.pop_off, geo, x, y, 0, fo);
+      if constexpr (PULL) local_store<S, T>(swap_all + d.pop_off, geo, x, y, 0, fo);
+      else push_store<S, T>(swap_all + d.pop_off, geo, x, y, 0, fo);
     }
     __syncthreads();
   }
@@ -355,6 +364,35 @@ k_batched_symmetrize(const double* __restrict__ n, T* __restrict__ pop, int L0, 
     for (int q = 0; q < 9; q++)
       pop[(size_t)q * V + g] = (T)(D2Q9::w[q] * (SHIFT ? v - rho0 : v));
     n_ref[g] = v;
+  }
+}
+
+// Conversion of every live domain between streamed populations and the pull
+// form (see k_restream): dir = +1 to the pull form, -1 back. Items tile the
+// live domains, W columns by Hs rows each, 256 threads per item.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_batched_restream(const T* __restrict__ src_all, T* __restrict__ dst_all,
+                   const BatchDom* __restrict__ doms, const BatchItem* __restrict__ items,
+                   const int* __restrict__ n_items, int W, int dir) {
+  const int ni = *n_items;
+  for (int it = blockIdx.x; it < ni; it += gridDim.x) {
+    const BatchItem w = items[it];
+    const BatchDom d = doms[w.dom];
+    const Geom geo{d.L0, d.L1, 1, (int64_t)d.L0 * d.L1};
+    const T* src = src_all + d.pop_off;
+    T* dst = dst_all + d.pop_off;
+    for (int k = threadIdx.x; k < W * w.Hs; k += 256) {
+      const int r = k / W, x = w.x0 + (k - r * W), y = w.m0 + r;
+      if (x >= d.L0 || y >= d.L1) continue;
+      const int64_t g = (int64_t)y * d.L0 + x;
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        const int xs = D2Q9::cx[q] ? wrap(x + dir * D2Q9::cx[q], d.L0) : x;
+        const int ys = D2Q9::cy[q] ? wrap(y + dir * D2Q9::cy[q], d.L1) : y;
+        dst[(size_t)q * geo.V + g] = src[(size_t)q * geo.V + (int64_t)ys * d.L0 + xs];
+      }
+    }
   }
 }
 

@@ -27,7 +27,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .configs import N_G, N_L, droplet
+from .configs import droplet
 
 PREC = {'fp64': 0, 'fp32s': 1}
 E_IDS = {'E4': 4, 'E6': 6, 'E8': 8}
@@ -84,7 +84,8 @@ class Batch:
             pass
 
     def set_launch(self, kind, tx, H, grid=0):
-        """kind 0: marching items of tx x H rows; kind 1: tx x H tiles."""
+        """kind 0: marching items of tx x H rows; kind 1: tx x H tiles;
+        kinds 2 / 3: 0 / 1 on the pull form."""
         _lib.call('idmx_batch_set_launch', self.handle, int(kind), int(tx), int(H), int(grid))
 
     def get_launch(self):

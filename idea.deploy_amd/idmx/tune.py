@@ -82,10 +82,10 @@ def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
 
 
 def batch_candidates(cus):
-    """(kind, tx, H, grid): marching items and tiles, three persistent grid
-    sizes each."""
-    shapes = [(0, tx, H) for tx in (64, 128, 256) for H in (8, 16, 32)]
-    shapes += [(1, 64, 4), (1, 32, 8), (1, 16, 16), (1, 32, 4)]
+    """(kind, tx, H, grid): marching items and tiles, push and pull form,
+    three persistent grid sizes each."""
+    shapes = [(k, tx, H) for k in (0, 2) for tx in (64, 128, 256) for H in (8, 16, 32)]
+    shapes += [(k, tx, H) for k in (1, 3) for tx, H in ((64, 4), (32, 8), (16, 16), (32, 4))]
     return [s + (g,) for s in shapes for g in (cus * 4, cus * 8, cus * 16)]
 
 
@@ -108,7 +108,9 @@ def tune_batch(tenet, points, precision, estencil='E4', steps=20, use_cache=True
     for cand in batch_candidates(cus):
         b.set_launch(*cand)
         b.run(2)
-        timings[','.join(map(str, cand))] = b.run(steps) / steps
+        t_short = b.run(2)
+        # difference of two run lengths: the pull form's conversions cancel
+        timings[','.join(map(str, cand))] = max(b.run(2 + steps) - t_short, 1e-6) / steps
     b.close()
     best = min(timings, key=timings.get)
     best = tuple(int(v) for v in best.split(','))

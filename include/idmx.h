@@ -245,8 +245,11 @@ int idmx_batch_destroy(idmx_batch_t b);
  * (64, 128, 256) by runs of H rows, each marched row by row with the psi
  * ring in LDS (large domains). kind 1 (tile, the default 32 x 8): items are
  * tx x H tiles (64x4, 32x8, 16x16, 32x4), one site per thread, halo psi in
- * LDS (small domains: no dependent row loads). grid = persistent blocks (0
- * keeps the current grid). Picked by the autotuner (idmx.tune). */
+ * LDS (small domains: no dependent row loads). kind 2 / 3: kind 0 / 1 run on
+ * the pull form (post-collision values at their own site, read from the
+ * source site: aligned stores); idmx_batch_run converts every live domain to
+ * that form before its steps and back after them. grid = persistent blocks
+ * (0 keeps the current grid). Picked by the autotuner (idmx.tune). */
 int idmx_batch_set_launch(idmx_batch_t b, int kind, int tx, int H, int grid);
 int idmx_batch_get_launch(idmx_batch_t b, int* kind, int* tx, int* H, int* grid,
                           int* n_items);

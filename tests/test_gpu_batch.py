@@ -39,7 +39,8 @@ def test_batch_matches_single_domain(tenet, precision, tol):
     starts = [cold_start(p) + 1e-4 * np.cos(np.arange(p.L * p.L)) for p in pts]
     refs = [_single(tenet, p, starts[i], 150, precision) for i, p in enumerate(pts)]
     for shape in [(0, 128, 32, 0), (0, 64, 16, 512), (0, 256, 64, 0), (1, 32, 8, 0),
-                  (1, 64, 4, 300), (1, 16, 16, 0), (1, 32, 4, 0)]:
+                  (1, 64, 4, 300), (1, 16, 16, 0), (1, 32, 4, 0), (2, 128, 32, 0),
+                  (2, 64, 8, 300), (3, 32, 8, 0), (3, 16, 16, 0)]:
         print('launch shape', shape, flush=True)
         b.set_launch(*shape)
         for i, n0 in enumerate(starts):
@@ -97,7 +98,8 @@ def test_batch_tiles_match_oracle_odd_sizes(tenet, es):
         cpuref.meta_run(st, 'D2' + es, p.G, p.tau, nsteps=40)
         refs.append(st.pop.reshape(9, -1).sum(axis=0))
     b = Batch(tenet, pts, 'fp64', estencil=es)
-    for shape in [(1, 32, 8, 0), (1, 64, 4, 0), (1, 16, 16, 7), (1, 32, 4, 0), (0, 64, 8, 0)]:
+    for shape in [(1, 32, 8, 0), (1, 64, 4, 0), (1, 16, 16, 7), (1, 32, 4, 0), (0, 64, 8, 0),
+                  (3, 32, 8, 0), (3, 64, 4, 5), (2, 64, 8, 0), (2, 128, 4, 0)]:
         print('es', es, 'launch shape', shape, flush=True)
         b.set_launch(*shape)
         for i, n0 in enumerate(starts):
