@@ -14,11 +14,12 @@ Modes
   FP32 + FP64: any start, FP32 shifted populations to tol32, then the state
                is converted to FP64 and relaxed to tol (FP64 fixes the final
                accuracy)
-A caller-supplied start whose residual stays above ten times its best value
-for `grow_checks` consecutive checks (or turns non-finite) is reset to the
-cold start once (the fallback of a bad warm start; the relaxation of a good
-start is not monotone either, so a smaller factor throws away good starts). A cold start is never reset:
-its trajectory would repeat.
+Fallback controller: a caller-supplied start that turns non-finite is reset
+to the cold start once; with grow_checks set, so is one whose residual stays
+above ten times its best value for that many consecutive checks. The growth
+rule is off by default: on a 128-point sweep it reset 40 learned starts and
+cost both time and converged points (the relaxation of a good start is not
+monotone). A cold start is never reset: its trajectory would repeat.
 """
 import ctypes as C
 import time
@@ -151,16 +152,17 @@ def _apply_launch(batch, active, launch, launch_small):
 
 
 def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks, hist,
-           restart, label, plateau=False, launch=None, launch_small=None):
+           restart, label, plateau=False, launch=None, launch_small=None, plateau_checks=5):
     """Runs the live domains of `batch` until each one's max |dn| < tol, or
     until every live domain has run max_steps steps in this call. A domain
-    on a caller-supplied start (warm[i]) whose residual stays above ten
-    times its best value for `grow_checks` consecutive checks, or turns
-    non-finite, is handed to restart(i) (the cold-start fallback) once. A
-    cold domain that turns non-finite is dropped as failed. With
-    plateau=True (the FP32 phase) a domain also finishes once its residual
-    has not improved by 20 % for `grow_checks` checks: FP32 rounding sets a
-    floor below which it cannot go, and the FP64 phase takes over there.
+    on a caller-supplied start (warm[i]) that turns non-finite, or (when
+    grow_checks is not None) whose residual stays above ten times its best
+    value for `grow_checks` consecutive checks, is handed to restart(i) (the
+    cold-start fallback) once. A cold domain that turns non-finite is dropped
+    as failed. With plateau=True (the FP32 phase) a domain also finishes
+    once its residual has not improved by 20 % for `plateau_checks` checks:
+    FP32 rounding sets a floor below which it cannot go, and the FP64 phase
+    takes over there.
     launch / launch_small: see _apply_launch.
     Returns (still active, failed, fell back, GPU ms)."""
     n = batch.n
@@ -186,10 +188,11 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
         best = np.where(active & fin, np.minimum(best, dn), best)
         done = active & fin & (dn < tol)
         if plateau:
-            done |= active & fin & (stale >= grow_checks)
+            done |= active & fin & (stale >= plateau_checks)
         hist.append((label, int(steps[active].max()), int(active.sum()),
                      float(np.max(np.where(active & fin, dn, 0.0)))))
-        bad = active & ~done & (~fin | (warm & (grows >= grow_checks)))
+        grew = grows >= grow_checks if grow_checks is not None else np.zeros(n, dtype=bool)
+        bad = active & ~done & (~fin | (warm & grew))
         dead = bad & ~warm
         failed |= dead
         changed = (done | dead).any()
@@ -209,7 +212,7 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
 
 
 def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
-              check_every=200, max_steps=400000, grow_checks=5, estencil='E4',
+              check_every=200, max_steps=400000, grow_checks=None, estencil='E4',
               launch=None, launch32=None, rho0=1.0, symmetric=True, launch_small=None,
               launch32_small=None):
     """Relax every point to steady state; returns SweepResult and the final
