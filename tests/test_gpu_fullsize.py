@@ -1,0 +1,100 @@
+"""Parity at the benchmark workloads' full sizes: the HIP step against the
+oracle on the 4096^2 E8 field with 64 droplets and on the 256^3 MRT sphere
+(a few steps: the oracle is serial), and the 128^2 droplet over its full
+10^4 steps through the Meta class. The launch is the one the autotuner
+picks for that workload on this device (push or pull form)."""
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+G = -5.0 / 3.0
+
+
+def _run_engine(tenet, eng, n0, nsteps):
+    from idmx.IdpyCode import IdpyMemory
+    from idmx.LBM._base import PopArray
+    n = IdpyMemory.OnDevice(n0, tenet)
+    u = IdpyMemory.Zeros(eng.V * eng.D, np.float64, tenet)
+    pop, swap = PopArray(eng, tenet), PopArray(eng, tenet)
+    eng.init_pop(pop, n, u)
+    m0, _ = eng.mass_momentum(pop)
+    if eng.run(pop, swap, nsteps):
+        pop, swap = swap, pop
+    m1, _ = eng.mass_momentum(pop)
+    f = eng.pop_to_host_soa(pop)
+    for x in (n, u, pop, swap):
+        x.Free()
+    return f, m0, m1
+
+
+def _oracle_pop(xi, es, dims, n0, nsteps, mrt=None):
+    from oracle import cpuref
+    st = cpuref.State(xi, dims)
+    cpuref.init_pop(st, n0, np.zeros(st.V * st.D))
+    cpuref.meta_run(st, es, G, 1.0, nsteps=nsteps, mrt=mrt)
+    return st.pop
+
+
+def test_d2q9_e8_4096_droplets_matches_oracle(tenet):
+    from idmx import configs, tune
+    from idmx.LBM._engine import SCEngine
+    dims = (4096, 4096)
+    n0 = configs.droplets2d()
+    eng = SCEngine(tenet, 'D2Q9', 'D2E8', dims, 'exp', G, 1.0)
+    best, _ = tune.tune_sim(eng, steps=3, use_cache=False)
+    print('launch', best, flush=True)
+    f, m0, m1 = _run_engine(tenet, eng, n0, 3)
+    eng.close()
+    ref = _oracle_pop('D2Q9', 'D2E8', dims, n0, 3)
+    err = rel_err(f.ravel(), ref)
+    print('rel err', err, 'mass drift', abs(m1 - m0) / m0, flush=True)
+    assert err <= 1e-12
+    assert abs(m1 - m0) <= 1e-13 * m0
+
+
+def test_d3q19_mrt_256_sphere_matches_oracle(tenet):
+    from idmx import configs, tune
+    from idmx.LBM import hermite
+    from idmx.LBM._engine import SCEngine
+    from oracle import hermite_mrt
+    import sympy as sp
+    dims = (256, 256, 256)
+    n0 = configs.sphere3d()
+    S = np.eye(19)
+    eng = SCEngine(tenet, 'D3Q19', 'D3E4', dims, 'exp', G, 1.0,
+                   mrt=hermite.mrt_kernel_tables('D3Q19', S))
+    best, _ = tune.tune_sim(eng, steps=3, use_cache=False)
+    print('launch', best, flush=True)
+    f, m0, m1 = _run_engine(tenet, eng, n0, 2)
+    eng.close()
+    ref = _oracle_pop('D3Q19', 'D3E4', dims, n0, 2,
+                      mrt=hermite_mrt.mrt_matrices('D3Q19', sp.eye(19)))
+    err = rel_err(f.ravel(), ref)
+    print('rel err', err, 'mass drift', abs(m1 - m0) / m0, flush=True)
+    assert err <= 1e-12
+    assert abs(m1 - m0) <= 1e-13 * m0
+
+
+def test_c1_droplet_full_length_meta_class(tenet):
+    """128^2 droplet with seeded noise, E4, 10^4 steps in checkpoints of
+    1000 (MainLoop), against the oracle's 10^4 steps."""
+    import sympy as sp
+    from idmx import configs
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=(128, 128), xi_stencil=XIStencils['D2Q9'],
+                             f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=G, tau=1.0)
+    n0 = configs.droplet2d_noise()
+    sim.SetDensityField(n0)
+    sim.MainLoop(list(range(0, 10001, 1000)))
+    f = sim.sims_idpy_memory['pop'].D2H()
+    sim.End()
+    ref = _oracle_pop('D2Q9', 'D2E4', (128, 128), n0, 10000)
+    err = rel_err(f.ravel(), ref)
+    print('rel err after 1e4 steps', err, flush=True)
+    assert err <= 1e-11
