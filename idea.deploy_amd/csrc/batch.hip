@@ -1,7 +1,9 @@
 // batch.hip -- batched Shan-Chen sweep over many D2Q9 domains (C ABI in
 // include/idmx.h, idmx_batch_*). The library owns the packed population
-// buffers; a step is one launch of k_batched over the live work items, and
-// idmx_batch_run replays a captured graph of an even number of such launches.
+// buffers; a step is one launch of k_batched (march items) or
+// k_batched_tile (tiles) over the live work items, in the push or the pull
+// form, and idmx_batch_run replays a captured graph of an even number of
+// such launches (pull form: between a conversion to it and one back).
 #include <algorithm>
 #include <cstring>
 #include <vector>

@@ -26,7 +26,6 @@ struct idmx_ctx_s {
   hipStream_t stream = nullptr;
   double* partials = nullptr;      // device scratch for reductions
   size_t partials_cap = 0;         // in doubles
-  double* host_result = nullptr;   // pinned, 16 doubles
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 

@@ -168,7 +168,6 @@ int idmx_ctx_create(int device, idmx_ctx_t* out) {
     c->partials_cap = 4 * 1024;
     e = hipMalloc(&c->partials, sizeof(double) * c->partials_cap);
   }
-  if (e == hipSuccess) e = hipHostMalloc(&c->host_result, sizeof(double) * 16, hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e != hipSuccess) {
@@ -184,7 +183,6 @@ int idmx_ctx_destroy(idmx_ctx_t c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->partials) hipFree(c->partials);
-  if (c->host_result) hipHostFree(c->host_result);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->stream) hipStreamDestroy(c->stream);
