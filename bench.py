@@ -326,6 +326,38 @@ def run_precision_study(steps=5000):
     return out
 
 
+def run_sweep_batch_only(n_points, Lmax, mode='fp64', max_steps=400000):
+    """The batched sweep alone at a larger scale (the per-point baseline
+    would take hours there): n_points centred droplets, L uniform in
+    {64, 96, ..., Lmax}, G in [-5.5, -4.2] (SC_G = G/3), R in [0.1 L, 0.35 L],
+    from the cold start to max |dn| < 1e-10 between checks 200 steps apart."""
+    from idmx import configs, sweep, tune
+    from idmx.IdpyCode import Tenet
+    tenet = Tenet(0)
+    pts = sweep.sample_points(n_points, configs.SEED0 + 6, Ls=tuple(range(64, Lmax + 1, 32)))
+    prec = 'fp64' if mode == 'fp64' else 'fp32s'
+    t0 = time.perf_counter()
+    l_full, tim = tune.tune_batch(tenet, pts, 'fp64', use_cache=False)
+    l_small = tune.tune_batch_small(tenet, pts, 'fp64')
+    kw = {'launch': l_full, 'launch_small': l_small}
+    if mode != 'fp64':
+        kw['launch32'], _ = tune.tune_batch(tenet, pts, 'fp32s', use_cache=False)
+        kw['launch32_small'] = tune.tune_batch_small(tenet, pts, 'fp32s')
+    t_tune = time.perf_counter() - t0
+    res, _ = sweep.run_sweep(tenet, pts, mode=mode, max_steps=max_steps, **kw,
+                             progress=lambda *h: print('progress', *h, file=sys.stderr,
+                                                       flush=True))
+    out = {'n_points': n_points, 'L': [64, Lmax], 'sites': int(sum(p.L * p.L for p in pts)),
+           'mode': mode, 'precision_batch': prec, 'tune_s': t_tune, 'launch': l_full,
+           'ms_per_step_all_live': tim[','.join(map(str, l_full))],
+           'launch_small': l_small, 'wall_s': res.wall_s, 'gpu_ms': res.gpu_ms,
+           'steps_mean': float(res.steps.mean()), 'steps_max': int(res.steps.max()),
+           'converged': int(res.converged.sum()),
+           'history': res.history[::max(1, len(res.history) // 16)] + res.history[-1:]}
+    tenet.End()
+    return out
+
+
 def calibrate(reps=10):
     """Plain copy of 19 x 256^3 float64 values (2.55 GB read + 2.55 GB
     written per launch): achievable HBM bandwidth and the reference launch
@@ -411,6 +443,10 @@ def main():
     ap.add_argument('--sweep', action='store_true',
                     help='run the (G, R, L) sweep study (time to steady state, learned '
                          'warm start vs plain) instead of the step benchmark')
+    ap.add_argument('--sweep-batch-only', type=int, default=0, metavar='N',
+                    help='time the batched sweep alone on N points (--sweep-Lmax, '
+                         '--sweep-mode)')
+    ap.add_argument('--sweep-mode', default='fp64', choices=['fp64', 'fp32+fp64'])
     ap.add_argument('--sweep-train', type=int, default=512)
     ap.add_argument('--sweep-eval', type=int, default=128)
     ap.add_argument('--sweep-Lmax', type=int, default=256)
@@ -430,6 +466,10 @@ def main():
         return
     if args.tune_report:
         tune_report(args.tune_report.split(','))
+        return
+    if args.sweep_batch_only:
+        r = run_sweep_batch_only(args.sweep_batch_only, args.sweep_Lmax, args.sweep_mode)
+        print(json.dumps({'sweep_batch_only': r}), flush=True)
         return
     if args.sweep:
         r = run_sweep_study(args.sweep_train, args.sweep_eval, args.sweep_Lmax,

@@ -152,7 +152,8 @@ def _apply_launch(batch, active, launch, launch_small):
 
 
 def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks, hist,
-           restart, label, plateau=False, launch=None, launch_small=None, plateau_checks=5):
+           restart, label, plateau=False, launch=None, launch_small=None, plateau_checks=5,
+           progress=None):
     """Runs the live domains of `batch` until each one's max |dn| < tol, or
     until every live domain has run max_steps steps in this call. A domain
     on a caller-supplied start (warm[i]) that turns non-finite, or (when
@@ -163,7 +164,8 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
     once its residual has not improved by 20 % for `plateau_checks` checks:
     FP32 rounding sets a floor below which it cannot go, and the FP64 phase
     takes over there.
-    launch / launch_small: see _apply_launch.
+    launch / launch_small: see _apply_launch. progress(label, steps, live,
+    max |dn|) is called every 50 checks.
     Returns (still active, failed, fell back, GPU ms)."""
     n = batch.n
     best = np.full(n, np.inf)
@@ -191,6 +193,8 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
             done |= active & fin & (stale >= plateau_checks)
         hist.append((label, int(steps[active].max()), int(active.sum()),
                      float(np.max(np.where(active & fin, dn, 0.0)))))
+        if progress is not None and len(hist) % 50 == 0:
+            progress(*hist[-1])
         grew = grows >= grow_checks if grow_checks is not None else np.zeros(n, dtype=bool)
         bad = active & ~done & (~fin | (warm & grew))
         dead = bad & ~warm
@@ -214,7 +218,7 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
 def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
               check_every=200, max_steps=400000, grow_checks=None, estencil='E4',
               launch=None, launch32=None, rho0=1.0, symmetric=True, launch_small=None,
-              launch32_small=None):
+              launch32_small=None, progress=None):
     """Relax every point to steady state; returns SweepResult and the final
     FP64 densities. start: None (cold) or a list of initial densities (None
     entries start cold). mode: 'fp64' (FP64 only) or 'fp32+fp64'.
@@ -267,7 +271,8 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
 
         _, failed, fb, ms = _relax(b32, np.ones(n, dtype=bool), warm, steps, tol32, check_every,
                                    max_steps, grow_checks, hist, restart32, 'fp32',
-                                   plateau=True, launch=launch32, launch_small=launch32_small)
+                                   plateau=True, launch=launch32, launch_small=launch32_small,
+                                   progress=progress)
         gpu_ms += ms
         fb_all |= fb
         # FP32 rounding leaves each domain a small net momentum that FP64
@@ -291,7 +296,7 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
         raise ValueError('mode is fp64 or fp32+fp64')
     rem, failed, fb, ms = _relax(b64, np.ones(n, dtype=bool), warm, steps, tol, check_every,
                                  max_steps, grow_checks, hist, restart64, 'fp64',
-                                 launch=launch, launch_small=launch_small)
+                                 launch=launch, launch_small=launch_small, progress=progress)
     gpu_ms += ms
     fb_all |= fb
     fail_all |= failed
