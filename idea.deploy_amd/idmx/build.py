@@ -22,7 +22,7 @@ ARCH = os.environ.get('IDMX_ARCH', 'gfx950')
 FLAGS = ['-std=c++17', '-O3', '-fPIC', '--offload-arch=' + ARCH,
          # packed-FP32 instructions are kept out of the generated code
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-         '-Wno-unused-result']
+         '-Wno-unused-result'] + os.environ.get('IDMX_HIPCC_FLAGS', '').split()
 
 SOURCES = ['runtime.hip', 'sc_d2q9.hip', 'sc_d3q19_e4.hip', 'sc_d3q19_e6.hip',
            'sc_d3q19_e8.hip', 'batch.hip', 'nn.hip']
