@@ -8,65 +8,15 @@
 #include <cstring>
 #include <vector>
 
-#include "batch_kernels.h"
-#include "internal.h"
+#include "batch_impl.h"
 
 using namespace idmx;
 
-struct idmx_batch_s {
-  idmx_ctx_s* ctx = nullptr;
-  int prec = 0, estencil = 4;
-  int kind = 1;  // 0: marching items (tx x H rows), 1: tiles (tx x H sites), 2/3: 0/1 on the
-                 // pull form (converted before and after each run)
-  int tx = 32, H = 8;
-  int ndom = 0;
-  std::vector<BatchDom> doms;
-  std::vector<int> active;
-  size_t elem = 8;
-  int64_t pop_elems = 0, sites = 0;
-  void* pop[2] = {nullptr, nullptr};
-  int cur = 0;  // index of the buffer holding the state
-  double* n_prev = nullptr;
-  double* stage = nullptr;  // per-domain FP64 staging (max V)
-  int64_t max_v = 0;
-  BatchDom* d_doms = nullptr;
-  BatchItem* d_items = nullptr;
-  int* d_nitems = nullptr;
-  unsigned long long* d_dmax = nullptr;
-  int max_items = 0, n_items = 0;
-  int grid = 0;
-  hipGraphExec_t graph = nullptr;
-  int graph_steps = 0;
-  double last_ms = 0;
-};
+hipError_t idmx::batch_step_f64(idmx_batch_s* b, const void* in, void* out) {
+  return step_es<double, false>(b, in, out);
+}
 
 namespace {
-
-template <class ES, class T, bool SHIFT, int TX, bool PULL>
-hipError_t launch_march(idmx_batch_s* b, const void* in, void* out) {
-  hipLaunchKernelGGL((k_batched<ES, T, SHIFT, TX, PULL>), dim3(b->grid), dim3(TX), 0,
-                     b->ctx->stream, (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems);
-  return hipGetLastError();
-}
-
-template <class ES, class T, bool SHIFT, int TX>
-hipError_t launch_step(idmx_batch_s* b, const void* in, void* out) {
-  return b->kind == 2 ? launch_march<ES, T, SHIFT, TX, true>(b, in, out)
-                      : launch_march<ES, T, SHIFT, TX, false>(b, in, out);
-}
-
-template <class ES, class T, bool SHIFT, int BX, int BY>
-hipError_t launch_tile(idmx_batch_s* b, const void* in, void* out) {
-  if (b->kind == 3)
-    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, true>), dim3(b->grid),
-                       dim3(BX * BY), 0, b->ctx->stream, (const T*)in, (T*)out, b->d_doms,
-                       b->d_items, b->d_nitems);
-  else
-    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, false>), dim3(b->grid),
-                       dim3(BX * BY), 0, b->ctx->stream, (const T*)in, (T*)out, b->d_doms,
-                       b->d_items, b->d_nitems);
-  return hipGetLastError();
-}
 
 // Streamed <-> pull form of every live domain (kinds 2 and 3).
 hipError_t restream(idmx_batch_s* b, int dir) {
@@ -89,33 +39,8 @@ bool tile_shape_ok(int tx, int H) {
          (tx == 32 && H == 4);
 }
 
-template <class ES, class T, bool SHIFT>
-hipError_t step_tx(idmx_batch_s* b, const void* in, void* out) {
-  if (b->kind == 1 || b->kind == 3) {
-    if (b->tx == 64) return launch_tile<ES, T, SHIFT, 64, 4>(b, in, out);
-    if (b->tx == 16) return launch_tile<ES, T, SHIFT, 16, 16>(b, in, out);
-    if (b->H == 4) return launch_tile<ES, T, SHIFT, 32, 4>(b, in, out);
-    return launch_tile<ES, T, SHIFT, 32, 8>(b, in, out);
-  }
-  switch (b->tx) {
-    case 64: return launch_step<ES, T, SHIFT, 64>(b, in, out);
-    case 256: return launch_step<ES, T, SHIFT, 256>(b, in, out);
-    default: return launch_step<ES, T, SHIFT, 128>(b, in, out);
-  }
-}
-
-template <class T, bool SHIFT>
-hipError_t step_es(idmx_batch_s* b, const void* in, void* out) {
-  switch (b->estencil) {
-    case IDMX_E6: return step_tx<D2E6, T, SHIFT>(b, in, out);
-    case IDMX_E8: return step_tx<D2E8, T, SHIFT>(b, in, out);
-    default: return step_tx<D2E4, T, SHIFT>(b, in, out);
-  }
-}
-
 hipError_t one_step(idmx_batch_s* b, const void* in, void* out) {
-  return b->prec == IDMX_PREC_F64 ? step_es<double, false>(b, in, out)
-                                  : step_es<float, true>(b, in, out);
+  return b->prec == IDMX_PREC_F64 ? batch_step_f64(b, in, out) : batch_step_f32(b, in, out);
 }
 
 template <class T, bool SHIFT>
@@ -300,8 +225,7 @@ int idmx_batch_set_density(idmx_batch_t b, int dom, const double* n_host) {
     hipLaunchKernelGGL((k_batched_init<double, false>), dim3(grid), dim3(256), 0, b->ctx->stream,
                        (double*)b->pop[b->cur] + D.pop_off, b->stage, V, 0.0);
   else
-    hipLaunchKernelGGL((k_batched_init<float, true>), dim3(grid), dim3(256), 0, b->ctx->stream,
-                       (float*)b->pop[b->cur] + D.pop_off, b->stage, V, D.rho0);
+    IDMX_HIP(batch_init_f32(b, (float*)b->pop[b->cur] + D.pop_off, b->stage, V, D.rho0, grid));
   IDMX_HIP(hipGetLastError());
   IDMX_HIP(hipMemcpyAsync(b->n_prev + D.site_off, b->stage, sizeof(double) * V,
                           hipMemcpyDeviceToDevice, b->ctx->stream));

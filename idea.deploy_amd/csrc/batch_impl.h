@@ -1,0 +1,100 @@
+// batch_impl.h -- state of one batched sweep (idmx_batch_t) and the step
+// launchers shared by batch.hip (host API, FP64 step) and batch_f32.hip
+// (FP32-shifted step). The FP64 unit is compiled without floating-point
+// contraction and the FP32 one with it, matching the single-domain kernels
+// of the same precision so that a batched domain follows the single-domain
+// trajectory.
+#pragma once
+
+#include <vector>
+
+#include "batch_kernels.h"
+#include "internal.h"
+
+struct idmx_batch_s {
+  idmx_ctx_s* ctx = nullptr;
+  int prec = 0, estencil = 4;
+  int kind = 1;  // 0: marching items (tx x H rows), 1: tiles (tx x H sites), 2/3: 0/1 on the
+                 // pull form (converted before and after each run)
+  int tx = 32, H = 8;
+  int ndom = 0;
+  std::vector<idmx::BatchDom> doms;
+  std::vector<int> active;
+  size_t elem = 8;
+  int64_t pop_elems = 0, sites = 0;
+  void* pop[2] = {nullptr, nullptr};
+  int cur = 0;  // index of the buffer holding the state
+  double* n_prev = nullptr;
+  double* stage = nullptr;  // per-domain FP64 staging (max V)
+  int64_t max_v = 0;
+  idmx::BatchDom* d_doms = nullptr;
+  idmx::BatchItem* d_items = nullptr;
+  int* d_nitems = nullptr;
+  unsigned long long* d_dmax = nullptr;
+  int max_items = 0, n_items = 0;
+  int grid = 0;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+  double last_ms = 0;
+};
+
+namespace idmx {
+
+template <class ES, class T, bool SHIFT, int TX, bool PULL>
+hipError_t launch_march(idmx_batch_s* b, const void* in, void* out) {
+  hipLaunchKernelGGL((k_batched<ES, T, SHIFT, TX, PULL>), dim3(b->grid), dim3(TX), 0,
+                     b->ctx->stream, (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems);
+  return hipGetLastError();
+}
+
+template <class ES, class T, bool SHIFT, int TX>
+hipError_t launch_step(idmx_batch_s* b, const void* in, void* out) {
+  return b->kind == 2 ? launch_march<ES, T, SHIFT, TX, true>(b, in, out)
+                      : launch_march<ES, T, SHIFT, TX, false>(b, in, out);
+}
+
+template <class ES, class T, bool SHIFT, int BX, int BY>
+hipError_t launch_tile(idmx_batch_s* b, const void* in, void* out) {
+  if (b->kind == 3)
+    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, true>), dim3(b->grid),
+                       dim3(BX * BY), 0, b->ctx->stream, (const T*)in, (T*)out, b->d_doms,
+                       b->d_items, b->d_nitems);
+  else
+    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, false>), dim3(b->grid),
+                       dim3(BX * BY), 0, b->ctx->stream, (const T*)in, (T*)out, b->d_doms,
+                       b->d_items, b->d_nitems);
+  return hipGetLastError();
+}
+
+template <class ES, class T, bool SHIFT>
+hipError_t step_tx(idmx_batch_s* b, const void* in, void* out) {
+  if (b->kind == 1 || b->kind == 3) {
+    if (b->tx == 64) return launch_tile<ES, T, SHIFT, 64, 4>(b, in, out);
+    if (b->tx == 16) return launch_tile<ES, T, SHIFT, 16, 16>(b, in, out);
+    if (b->H == 4) return launch_tile<ES, T, SHIFT, 32, 4>(b, in, out);
+    return launch_tile<ES, T, SHIFT, 32, 8>(b, in, out);
+  }
+  switch (b->tx) {
+    case 64: return launch_step<ES, T, SHIFT, 64>(b, in, out);
+    case 256: return launch_step<ES, T, SHIFT, 256>(b, in, out);
+    default: return launch_step<ES, T, SHIFT, 128>(b, in, out);
+  }
+}
+
+template <class T, bool SHIFT>
+hipError_t step_es(idmx_batch_s* b, const void* in, void* out) {
+  switch (b->estencil) {
+    case IDMX_E6: return step_tx<D2E6, T, SHIFT>(b, in, out);
+    case IDMX_E8: return step_tx<D2E8, T, SHIFT>(b, in, out);
+    default: return step_tx<D2E4, T, SHIFT>(b, in, out);
+  }
+}
+
+// One step of every live domain, in the FP64 and the FP32-shifted precision.
+hipError_t batch_step_f64(idmx_batch_s* b, const void* in, void* out);
+hipError_t batch_step_f32(idmx_batch_s* b, const void* in, void* out);
+// FP32-shifted populations of one domain at rest equilibrium of density n.
+hipError_t batch_init_f32(idmx_batch_s* b, float* pop, const double* n, int64_t V, double rho0,
+                          unsigned grid);
+
+}  // namespace idmx

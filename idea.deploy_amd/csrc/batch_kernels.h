@@ -397,7 +397,7 @@ k_batched_restream(const T* __restrict__ src_all, T* __restrict__ dst_all,
 }
 
 // Per-block partial sums of n over V sites (FP64).
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_batched_sum_partial(const double* __restrict__ n, int64_t V, double* __restrict__ out) {
   double s = 0.0;
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;

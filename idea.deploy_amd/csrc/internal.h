@@ -60,7 +60,11 @@ struct SimOps {
   int (*from_f64)(idmx_sc_s*, void* pop, const double* src);
 };
 
-const SimOps* get_ops_d2q9(int estencil, int precision, int coll);
-const SimOps* get_ops_d3q19_e4(int precision, int coll);
-const SimOps* get_ops_d3q19_e6(int precision, int coll);
-const SimOps* get_ops_d3q19_e8(int precision, int coll);
+const SimOps* get_ops_d2q9_f64(int estencil, int coll);
+const SimOps* get_ops_d2q9_f32(int estencil, int coll);
+const SimOps* get_ops_d3q19_e4_f64(int coll);
+const SimOps* get_ops_d3q19_e4_f32(int coll);
+const SimOps* get_ops_d3q19_e6_f64(int coll);
+const SimOps* get_ops_d3q19_e6_f32(int coll);
+const SimOps* get_ops_d3q19_e8_f64(int coll);
+const SimOps* get_ops_d3q19_e8_f32(int coll);

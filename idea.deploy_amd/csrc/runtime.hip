@@ -364,10 +364,15 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
   s->geo.L2 = D == 3 ? d->dims[2] : 1;
   s->geo.V = V;
   s->pop_elem = d->precision == IDMX_PREC_F64 ? 8 : 4;
-  if (D == 2) s->ops = get_ops_d2q9(d->estencil, d->precision, s->coll);
-  else if (d->estencil == IDMX_E4) s->ops = get_ops_d3q19_e4(d->precision, s->coll);
-  else if (d->estencil == IDMX_E6) s->ops = get_ops_d3q19_e6(d->precision, s->coll);
-  else s->ops = get_ops_d3q19_e8(d->precision, s->coll);
+  const bool f64 = d->precision == IDMX_PREC_F64;
+  if (D == 2)
+    s->ops = f64 ? get_ops_d2q9_f64(d->estencil, s->coll) : get_ops_d2q9_f32(d->estencil, s->coll);
+  else if (d->estencil == IDMX_E4)
+    s->ops = f64 ? get_ops_d3q19_e4_f64(s->coll) : get_ops_d3q19_e4_f32(s->coll);
+  else if (d->estencil == IDMX_E6)
+    s->ops = f64 ? get_ops_d3q19_e6_f64(s->coll) : get_ops_d3q19_e6_f32(s->coll);
+  else
+    s->ops = f64 ? get_ops_d3q19_e8_f64(s->coll) : get_ops_d3q19_e8_f32(s->coll);
   if (!s->ops) {
     delete s;
     set_error("idmx_sc_create: unsupported combination");

@@ -1,6 +1,7 @@
-// Instantiates the D3Q19 kernels for the E8 stencil.
+// Instantiates the FP64 D3Q19 kernels for the E8 stencil (compiled without
+// floating-point contraction).
 #include "sc_impl.h"
 
-const SimOps* get_ops_d3q19_e8(int precision, int coll) {
-  return idmx::ops_for<D3Q19, D3E8>(precision, coll);
+const SimOps* get_ops_d3q19_e8_f64(int coll) {
+  return idmx::ops_for_f64<D3Q19, D3E8>(coll);
 }

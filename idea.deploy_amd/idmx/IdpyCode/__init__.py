@@ -67,8 +67,11 @@ class Tenet:
 
     def End(self):
         if self.handle:
+            # arrays, simulations and batches still open on this context are
+            # released first: their handles are not valid once it is gone
             for a in list(self._live.values()):
-                a.Free()
+                a.close() if hasattr(a, 'close') else a.Free()
+            self._live.clear()
             _lib.call('idmx_ctx_destroy', self.handle)
             self.handle = None
 

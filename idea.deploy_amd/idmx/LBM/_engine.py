@@ -56,13 +56,16 @@ class SCEngine:
         h = C.c_void_p()
         _lib.call('idmx_sc_create', tenet.handle, C.byref(d), C.byref(h))
         self.handle = h
+        tenet._live[id(self)] = self
         nb = C.c_size_t()
         _lib.call('idmx_sc_pop_bytes', h, C.byref(nb))
         self.pop_bytes = nb.value
 
     def close(self):
         if self.handle:
-            _lib.call('idmx_sc_destroy', self.handle)
+            if self.tenet.handle:
+                _lib.call('idmx_sc_destroy', self.handle)
+            self.tenet._live.pop(id(self), None)
             self.handle = None
 
     def __del__(self):

@@ -71,11 +71,14 @@ class Batch:
         _lib.call('idmx_batch_create', tenet.handle, arr, len(self.points), PREC[precision],
                   E_IDS[estencil], 0, C.byref(h))
         self.handle = h
+        tenet._live[id(self)] = self
         self.n = len(self.points)
 
     def close(self):
         if self.handle:
-            _lib.call('idmx_batch_destroy', self.handle)
+            if self.tenet.handle:
+                _lib.call('idmx_batch_destroy', self.handle)
+            self.tenet._live.pop(id(self), None)
             self.handle = None
 
     def __del__(self):

@@ -100,6 +100,8 @@ int idmx_device_count(int* count);
 /* Creates a context on device `device` with its own HIP stream. Replaces the
  * CTypes/CUDA Tenet (idpy/CTypes/CTypes.py:59-83, idpy/CUDA/CUDA.py:46). */
 int idmx_ctx_create(int device, idmx_ctx_t* out);
+/* Destroys the context and its stream. Simulations and batches created on it
+ * must be destroyed first; their handles are invalid afterwards. */
 int idmx_ctx_destroy(idmx_ctx_t ctx);
 int idmx_sync(idmx_ctx_t ctx);
 /* Device name into buf (the 'device_name' entry of IdpyLoopProfile,
