@@ -58,6 +58,7 @@ struct SimOps {
   int (*mass_mom)(idmx_sc_s*, const void* pop, double* mass, double* mom);
   int (*to_f64)(idmx_sc_s*, const void* pop, double* dst);
   int (*from_f64)(idmx_sc_s*, void* pop, const double* src);
+  int (*launch_ok)(int tx, int ty, int variant);
 };
 
 const SimOps* get_ops_d2q9_f64(int estencil, int coll);

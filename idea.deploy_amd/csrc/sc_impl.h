@@ -55,14 +55,38 @@ struct Impl {
     return hipGetLastError();
   }
 
+  // Lead plane staged in LDS by DMA (variant 9); tiles whose staging buffer
+  // does not fit the LDS of a CU are refused.
+  template <int TX, int TY>
+  static hipError_t launch_dma(idmx_sc_s* s, const T* pop, T* swap) {
+    if constexpr (DmaShape<S, ES, T, TX, TY, COLL>::fits) {
+      const Geom& g = s->geo;
+      const int ntx = (g.L0 + TX - 1) / TX;
+      const int nty = S::D == 3 ? (g.L1 + TY - 1) / TY : 1;
+      const int Lm = S::D == 3 ? g.L2 : g.L1;
+      const int H = std::min(s->H, Lm);
+      const int nseg = (Lm + H - 1) / H;
+      const int ntiles = ntx * nty * nseg;
+      const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
+      hipLaunchKernelGGL((k_fused_dma<S, ES, T, SHIFT, COLL, TX, TY>), dim3(grid),
+                         dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H, ntx, nty,
+                         ntiles);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidConfiguration;
+    }
+  }
+
   // variant: 1 populations held in registers, 0 re-read, 2 re-read with
   // prefetch of the next lead plane, 3 re-read with >= 3 waves/SIMD,
   // 4 re-read with prefetch and >= 3 waves/SIMD; 5 and 6 are 1 and 0 on the
   // pull form (run() converts to and from it around the steps); 7 and 8
-  // are 0 and 6 with two lead planes per barrier
+  // are 0 and 6 with two lead planes per barrier; 9 stages the lead plane in
+  // LDS by DMA (k_fused_dma)
   template <int TX, int TY>
   static hipError_t variant(idmx_sc_s* s, const T* pop, T* swap) {
     switch (s->hist) {
+      case 9: return launch_dma<TX, TY>(s, pop, swap);
       case 7: return launch_fused<TX, TY, false, false, false, false, false, true>(
           s, pop, swap, nullptr, nullptr, nullptr);
       case 8: return launch_fused<TX, TY, false, false, false, false, true, true>(
@@ -263,9 +287,29 @@ struct Impl {
     return IDMX_OK;
   }
 
+  // Whether launch variant v can run the tile tx x ty (the LDS staging of
+  // variant 9 has to fit one CU).
+  static int launch_ok(int tx, int ty, int v) {
+    if (v != 9) return 1;
+    if constexpr (S::D == 2) {
+      switch (tx) {
+        case 64: return DmaShape<S, ES, T, 64, 1, COLL>::fits;
+        case 128: return DmaShape<S, ES, T, 128, 1, COLL>::fits;
+        default: return DmaShape<S, ES, T, 256, 1, COLL>::fits;
+      }
+    } else {
+      switch (tx * 100 + ty) {
+        case 6404: return DmaShape<S, ES, T, 64, 4, COLL>::fits;
+        case 12802: return DmaShape<S, ES, T, 128, 2, COLL>::fits;
+        case 25602: return DmaShape<S, ES, T, 256, 2, COLL>::fits;
+        default: return DmaShape<S, ES, T, 32, 8, COLL>::fits;
+      }
+    }
+  }
+
   static const SimOps* ops() {
     static const SimOps o = {run, density_psi, fcs, moments, density_velocity, init_pop,
-                             mass_mom, to_f64, from_f64};
+                             mass_mom, to_f64, from_f64, launch_ok};
     return &o;
   }
 };

@@ -117,6 +117,18 @@ class SCEngine:
     def set_launch(self, tx, ty, H, hist=1):
         _lib.call('idmx_sc_set_launch', self.handle, int(tx), int(ty), int(H), int(hist))
 
+    def launch_ok(self, tx, ty, H, hist=1):
+        """Whether set_launch accepts this shape (variant 9 needs its LDS
+        staging buffer to fit one CU); the current launch is kept."""
+        old = self.get_launch()
+        try:
+            self.set_launch(tx, ty, H, hist)
+            return True
+        except _lib.IdmxError:
+            return False
+        finally:
+            self.set_launch(*old)
+
     def get_launch(self):
         v = [C.c_int() for _ in range(4)]
         _lib.call('idmx_sc_get_launch', self.handle, *[C.byref(x) for x in v])

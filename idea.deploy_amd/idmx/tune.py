@@ -35,7 +35,8 @@ def _store(key, val):
 def sim_candidates(D, L_march):
     tiles = ((64, 1), (128, 1), (256, 1)) if D == 2 else ((32, 8), (64, 4), (128, 2), (256, 2))
     Hs = [h for h in (32, 64, 128) if h <= L_march] or [L_march]
-    return [(tx, ty, h, var) for tx, ty in tiles for h in Hs for var in (1, 0, 2, 3, 4, 5, 6, 7, 8)]
+    return [(tx, ty, h, var) for tx, ty in tiles for h in Hs
+            for var in (1, 0, 2, 3, 4, 5, 6, 7, 8, 9)]
 
 
 def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
@@ -65,6 +66,8 @@ def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
     timings = {}
     old = engine.get_launch()
     for cand in sim_candidates(D, engine.dims[-1]):
+        if not engine.launch_ok(*cand):
+            continue
         engine.set_launch(*cand)
         engine.run(a, b, 2)
         t_short = engine.last_run_ms()

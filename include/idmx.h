@@ -214,7 +214,10 @@ int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
  * read from the source site; idmx_sc_run converts to that form before the
  * steps and back after them, so pop always holds streamed populations
  * between calls), 7 and 8 are 0 and 6 with two lead planes loaded per
- * barrier. Chosen at creation;
+ * barrier, 9 holds them in registers and copies each lead plane (tile and
+ * halo) into LDS by DMA while the previous plane collides (refused with
+ * IDMX_E_ARG for tiles whose staging buffer exceeds the 160 KiB of LDS of a
+ * CU, e.g. FP64 256x2). Chosen at creation;
  * the autotuner (idmx.tune) overrides it. No reference counterpart: the
  * reference fixes block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
 int idmx_sc_set_launch(idmx_sc_t sim, int tx, int ty, int H, int hist);

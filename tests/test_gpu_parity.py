@@ -155,7 +155,7 @@ def test_every_launch_variant_matches_oracle(tenet, xi, es, dims):
     n0 = droplet_field(dims, seed=21)
     f_ref, _, _, _ = _oracle(xi, es, dims, n0, 12)
     eng = SCEngine(tenet, xi, es, dims, 'exp', G, 1.0)
-    cands = tune.sim_candidates(len(dims), dims[-1])
+    cands = [c for c in tune.sim_candidates(len(dims), dims[-1]) if eng.launch_ok(*c)]
     # the tuner's march lengths exceed these small domains, so also march
     # short segments (segment boundaries, a last shorter segment)
     for cand in cands + [c[:2] + (3,) + c[3:] for c in cands] + [
@@ -169,6 +169,55 @@ def test_every_launch_variant_matches_oracle(tenet, xi, es, dims):
         for x in (pop, swap, n, psi, u):
             x.Free()
     eng.close()
+
+
+@pytest.mark.parametrize('xi,es,dims,prec,coll', [
+    ('D3Q19', 'D3E4', (70, 19, 9), 'fp64', 'diag'), ('D3Q19', 'D3E4', (70, 19, 9), 'fp64', 'dense'),
+    ('D3Q19', 'D3E4', (70, 19, 9), 'fp32s', 'diag'), ('D3Q19', 'D3E4', (71, 19, 9), 'fp32s', 'bgk'),
+    ('D3Q19', 'D3E8', (35, 17, 7), 'fp64', 'diag'), ('D2Q9', 'D2E8', (300, 17), 'fp64', 'diag'),
+    ('D2Q9', 'D2E4', (37, 29), 'fp32s', 'bgk')])
+def test_lds_staged_variant_matches_oracle(tenet, xi, es, dims, prec, coll):
+    """Variant 9 (lead plane copied into LDS by DMA while the previous plane
+    collides) with BGK, diagonal and dense MRT, FP64 and FP32-shifted, on
+    sizes that leave partial tiles and a short last segment, against the
+    oracle; every tile shape whose staging buffer fits a CU."""
+    import sympy as sp
+    from idmx import tune
+    from idmx.LBM import hermite
+    from idmx.LBM._engine import SCEngine
+    from oracle import hermite_mrt
+    Q = 9 if xi == 'D2Q9' else 19
+    mats, tables = None, None
+    if coll != 'bgk':
+        rng = np.random.default_rng(6)
+        s = rng.uniform(0.6, 1.6, size=Q)
+        # dense: the (A, beta, gamma) tables of the same diagonal S
+        tables = (hermite.mrt_kernel_tables(xi, np.diag(s)) if coll == 'diag'
+                  else hermite.mrt_tables(xi, np.diag(s)))
+        assert isinstance(tables, dict) == (coll == 'diag')
+        mats = hermite_mrt.mrt_matrices(
+            xi, sp.diag(*[sp.Rational(v).limit_denominator(10 ** 15) for v in s]))
+    n0 = droplet_field(dims, seed=33)
+    f_ref, _, _, _ = _oracle(xi, es, dims, n0, 12, mrt=mats)
+    eng = SCEngine(tenet, xi, es, dims, 'exp', G, 1.0, mrt=tables, precision=prec, rho0=1.0)
+    cands = [c for c in tune.sim_candidates(len(dims), dims[-1]) if c[3] == 9]
+    cands += [c[:2] + (3, 9) for c in cands]
+    tol = 1e-11 if prec == 'fp64' else 5e-6
+    ran = 0
+    for cand in cands:
+        if not eng.launch_ok(*cand):
+            continue
+        print('variant', cand, flush=True)
+        eng.set_launch(*cand)
+        pop, swap, n, psi, u = _device_fields(tenet, eng, n0)
+        if eng.run(pop, swap, 12):
+            pop, swap = swap, pop
+        assert rel_err(eng.pop_to_host_soa(pop).ravel(), f_ref) <= tol, cand
+        for x in (pop, swap, n, psi, u):
+            x.Free()
+        ran += 1
+    eng.close()
+    assert ran >= 2
 
 
 @pytest.mark.parametrize('xi,es,dims,prec', [('D2Q9', 'D2E4', (24, 21), 'fp64'),
