@@ -463,6 +463,12 @@ int idmx_sc_set_launch(idmx_sc_t s, int tx, int ty, int H, int hist) {
   return IDMX_OK;
 }
 
+int idmx_sc_set_walls(idmx_sc_t s, const void* walls) {
+  CHECK_ARG(s, "idmx_sc_set_walls: null simulation");
+  s->walls = (const uint8_t*)walls;
+  return IDMX_OK;
+}
+
 int idmx_sc_get_launch(idmx_sc_t s, int* tx, int* ty, int* H, int* hist) {
   CHECK_ARG(s && tx && ty && H && hist, "idmx_sc_get_launch: invalid argument");
   *tx = s->tx;

@@ -142,22 +142,25 @@ struct Impl {
                              double* psi, double* u, bool wf) {
     const unsigned grid = grid_for(s->geo.V);
     hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid), dim3(256), 0,
-                       s->ctx->stream, pop, s->geo, phys(s), n, psi);
+                       s->ctx->stream, pop, s->geo, phys(s), n, psi, s->walls);
     if (wf)
       hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, true>), dim3(grid), dim3(256), 0,
-                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u);
+                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u, s->walls);
     else
       hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, false>), dim3(grid), dim3(256), 0,
-                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u);
+                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u, s->walls);
     return hipGetLastError();
   }
 
   static int run(idmx_sc_s* s, void** pop, void** swap, double* n, double* psi,
                  double* u, int nsteps, int flags) {
     const bool wf = (flags & IDMX_RUN_WRITE_FIELDS) != 0;
-    const bool twop = (flags & IDMX_RUN_TWO_PASS) != 0;
+    // with walls the step is the two-pass one (density and psi of the fluid
+    // sites into n / psi, where psi also holds the solid sites' values)
+    const bool twop = (flags & IDMX_RUN_TWO_PASS) != 0 || s->walls;
     if (twop && (!n || !psi)) {
-      set_error("idmx_sc_run: the two-pass step needs n and psi buffers");
+      set_error(s->walls ? "idmx_sc_run: a simulation with walls needs n and psi buffers"
+                         : "idmx_sc_run: the two-pass step needs n and psi buffers");
       return IDMX_E_ARG;
     }
     if (wf && (!n || !psi || !u)) {
@@ -203,7 +206,7 @@ struct Impl {
 
   static int density_psi(idmx_sc_s* s, double* n, double* psi, const void* pop) {
     hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid_for(s->geo.V)), dim3(256),
-                       0, s->ctx->stream, (const T*)pop, s->geo, phys(s), n, psi);
+                       0, s->ctx->stream, (const T*)pop, s->geo, phys(s), n, psi, s->walls);
     IDMX_HIP(hipGetLastError());
     return IDMX_OK;
   }
@@ -213,11 +216,12 @@ struct Impl {
     const unsigned grid = grid_for(s->geo.V);
     if (u)
       hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, true>), dim3(grid), dim3(256), 0,
-                         s->ctx->stream, (const T*)pop, (T*)swap, n, psi, s->geo, phys(s), u);
+                         s->ctx->stream, (const T*)pop, (T*)swap, n, psi, s->geo, phys(s), u,
+                         s->walls);
     else
       hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, false>), dim3(grid), dim3(256), 0,
                          s->ctx->stream, (const T*)pop, (T*)swap, n, psi, s->geo, phys(s),
-                         nullptr);
+                         nullptr, s->walls);
     IDMX_HIP(hipGetLastError());
     return IDMX_OK;
   }
@@ -225,9 +229,10 @@ struct Impl {
   static int moments(idmx_sc_s* s, const void* pop, double* n, double* psi, double* u) {
     const unsigned grid = grid_for(s->geo.V);
     hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid), dim3(256), 0,
-                       s->ctx->stream, (const T*)pop, s->geo, phys(s), n, psi);
+                       s->ctx->stream, (const T*)pop, s->geo, phys(s), n, psi, s->walls);
     hipLaunchKernelGGL((k_velocity<S, ES, T, SHIFT>), dim3(grid), dim3(256), 0,
-                       s->ctx->stream, (const T*)pop, n, psi, s->geo, s->desc.SC_G, u);
+                       s->ctx->stream, (const T*)pop, n, psi, s->geo, s->desc.SC_G, u,
+                       s->walls);
     IDMX_HIP(hipGetLastError());
     return IDMX_OK;
   }
@@ -236,10 +241,10 @@ struct Impl {
   static int density_velocity(idmx_sc_s* s, const void* pop, double* n, double* u) {
     const unsigned grid = grid_for(s->geo.V);
     hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid), dim3(256), 0,
-                       s->ctx->stream, (const T*)pop, s->geo, phys(s), n, nullptr);
+                       s->ctx->stream, (const T*)pop, s->geo, phys(s), n, nullptr, nullptr);
     // G = 0: no force term; n stands in for the (unused) psi argument
     hipLaunchKernelGGL((k_velocity<S, ES, T, SHIFT>), dim3(grid), dim3(256), 0,
-                       s->ctx->stream, (const T*)pop, n, n, s->geo, 0.0, u);
+                       s->ctx->stream, (const T*)pop, n, n, s->geo, 0.0, u, nullptr);
     IDMX_HIP(hipGetLastError());
     return IDMX_OK;
   }

@@ -79,13 +79,56 @@ class ShanChenMultiPhase(_ShanChenCommon):
         return self._run_loop(time_steps, convergence_functions, profiling, eng,
                               self._u_step)
 
+    def MainLoopWalls(self, time_steps, convergence_functions=[], profiling=False):
+        """MainLoopWalls (MultiPhase.py:1226-1309): the Meta step on the fluid
+        sites of the walls set by GetWalls, push streaming with half-way
+        bounce-back at the solid sites, whose psi (GetWalls' psi_w) enters
+        the force of their fluid neighbours. Runs the two-pass step."""
+        m = self.sims_idpy_memory
+        if 'walls' not in m:
+            raise Exception("Missing 'walls': call GetWalls first")
+        self.engine.set_walls(m['walls'])
+        try:
+            return self._run_loop(time_steps, convergence_functions, profiling, self.engine,
+                                  self._u_step)
+        finally:
+            self.engine.set_walls(None)
+
+    def GetWalls(self, walls, psi_w):
+        """psi_w (V values, the psi of the solid sites; the fluid sites' are
+        recomputed every step) and the walls flags (1 fluid, 0 solid, shape
+        flip(dim_sizes)) to the device (MultiPhase.py:157-161, LBM.py:609-631)."""
+        V = self.sims_vars['V']
+        psi_w = np.ravel(np.asarray(psi_w, dtype=np.float64))
+        if psi_w.size != V:
+            raise ValueError('psi_w must hold V = %d values' % V)
+        self.sims_idpy_memory['psi'].H2D(psi_w)
+        m = self.sims_idpy_memory
+        if 'xi_opposite' not in m:
+            xi = np.array(self.params_dict['xi_stencil']['XIs'])
+            opp = [int(np.where((xi == -c).all(axis=1))[0][0]) for c in xi]
+            m['xi_opposite'] = IdpyMemory.OnDevice(np.array(opp, dtype=np.int32), self.tenet)
+        if 'walls' not in m:
+            m['walls'] = IdpyMemory.OnDevice(np.ones(V, dtype=np.uint8), self.tenet)
+        dims_rev = list(self.sims_vars['dim_sizes'])[::-1]
+        walls = np.asarray(walls)
+        if list(walls.shape) == dims_rev:
+            m['walls'].H2D(np.ravel(walls).astype(np.uint8))
+
     def ComputeMomentsMeta(self):
         """n, psi and the Guo-shifted u = sum xi f / n + F/(2n) on the device
-        (MultiPhase.py:1565-1662)."""
+        (MultiPhase.py:1565-1662); with walls, on the fluid sites only."""
         if not self.init_status['pop']:
             raise Exception('Populations are not initialized')
         m = self.sims_idpy_memory
-        self.engine.moments(m['pop'], m['n'], m['psi'], m['u'])
+        walls = m.get('walls')
+        if walls is not None:
+            self.engine.set_walls(walls)
+        try:
+            self.engine.moments(m['pop'], m['n'], m['psi'], m['u'])
+        finally:
+            if walls is not None:
+                self.engine.set_walls(None)
         self.init_status['n'] = True
         self.init_status['u'] = True
 

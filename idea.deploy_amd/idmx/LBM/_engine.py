@@ -117,6 +117,11 @@ class SCEngine:
     def set_launch(self, tx, ty, H, hist=1):
         _lib.call('idmx_sc_set_launch', self.handle, int(tx), int(ty), int(H), int(hist))
 
+    def set_walls(self, walls):
+        """walls: device array of V uint8 flags (1 fluid, 0 solid), or None.
+        While set, run / moments step as MainLoopWalls."""
+        _lib.call('idmx_sc_set_walls', self.handle, walls.ptr if walls is not None else None)
+
     def launch_ok(self, tx, ty, H, hist=1):
         """Whether set_launch accepts this shape (variant 9 needs its LDS
         staging buffer to fit one CU); the current launch is kept."""

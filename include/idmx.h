@@ -221,6 +221,18 @@ int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
  * the autotuner (idmx.tune) overrides it. No reference counterpart: the
  * reference fixes block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
 int idmx_sc_set_launch(idmx_sc_t sim, int tx, int ty, int H, int hist);
+
+/* Walls: `walls` is a device array of V unsigned chars (the reference's
+ * FlagType), 1 = fluid, 0 = solid; NULL removes them. The simulation borrows
+ * it. With walls, idmx_sc_run steps as MainLoopWalls
+ * (idpy/LBM/MultiPhase.py:1226-1309): K_ComputeDensityPsiWallsMeta on the
+ * fluid sites, then K_ForceCollideStreamWallsSCMPMeta (fluid sites, push
+ * with half-way bounce-back at solid sites;
+ * idpy/LBM/MultiPhaseKernelsMeta.py:42-104, 524-716), and needs the n and psi
+ * buffers: psi at solid sites keeps what the caller stored (GetWalls,
+ * MultiPhase.py:157-161). idmx_sc_density_psi, idmx_sc_force_collide_stream
+ * and idmx_sc_moments honour the walls the same way. */
+int idmx_sc_set_walls(idmx_sc_t sim, const void* walls);
 int idmx_sc_get_launch(idmx_sc_t sim, int* tx, int* ty, int* H, int* hist);
 
 /* Milliseconds of the last idmx_sc_run, measured with HIP events recorded

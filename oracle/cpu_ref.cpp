@@ -23,6 +23,12 @@
 //     f* = M^-1 [S M f_eq + (1-S) M f] + M'^-1 (1-S/2) M' F_guo
 //     (idpy/LBM/Collisions.py:96-160, idpy/LBM/Forcings.py:129-181); the three
 //     Q x Q products are formed once on the host by oracle/hermite_mrt.py.
+//   * MainLoopWalls (idpy/LBM/MultiPhase.py:1226-1309): the same step on the
+//     fluid sites only (walls[g] == 1), K_ComputeDensityPsiWallsMeta
+//     (MultiPhaseKernelsMeta.py:42-104) and K_ForceCollideStreamWallsSCMPMeta
+//     (:524-716), whose push stream reflects a population aimed at a solid
+//     site (walls == 0) back into the opposite direction at its source site
+//     (SRTCollisionPlusGuoPushStreamWallsCode, idpy/LBM/Collisions.py:835-850);
 //   * ComputeMomentsMeta (idpy/LBM/MultiPhase.py:1565-1662,
 //     K_ComputeVelocityAfterForceSCMPMeta MultiPhaseKernelsMeta.py:1077-1219)
 //   * InitPopulations (K_InitPopulations idpy/LBM/LBM.py:1700-1743)
@@ -220,22 +226,36 @@ struct MRTMats {
   const double* Gg;  // M'^-1 (1-S/2) M'
 };
 
+// Index of the opposite velocity (IdpyStencil.opposite).
+template <class S>
+int opposite(int q) {
+  for (int k = 0; k < S::Q; k++) {
+    bool same = true;
+    for (int d = 0; d < 3; d++) same = same && S::xi[k][d] == -S::xi[q][d];
+    if (same) return k;
+  }
+  return q;
+}
+
 template <class S>
 void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
                double omega, int psi_kind, const MRTMats* mrt, double*& pop,
-               double*& pop_swap, double* n, double* psi) {
+               double*& pop_swap, double* n, double* psi, const uint8_t* walls) {
   constexpr int D = S::D, Q = S::Q;
   const int64_t V = geo.V;
-  // K_ComputeDensityPsiMeta (MultiPhaseKernelsMeta.py:107-167)
+  // K_ComputeDensityPsiMeta (MultiPhaseKernelsMeta.py:107-167); with walls
+  // K_ComputeDensityPsiWallsMeta (:42-104), fluid sites only
   for (int64_t g = 0; g < V; g++) {
+    if (walls && walls[g] != 1) continue;
     double ln = 0.;
     for (int q = 0; q < Q; q++) ln += pop[ord(g, q)];
     psi[g] = psi_of(psi_kind, ln);
     n[g] = ln;
   }
   // K_ForceCollideStreamSCMPMeta / _MRTMeta (MultiPhaseKernelsMeta.py:169-345,
-  // 891-1075)
+  // 891-1075); with walls K_ForceCollideStreamWallsSCMPMeta (:524-716)
   for (int64_t g = 0; g < V; g++) {
+    if (walls && walls[g] != 1) continue;
     double F[3];
     sc_force<D>(geo, es, psi, g, G, F);
     const double ln = n[g];
@@ -268,12 +288,20 @@ void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
         fout[q] = a;
       }
     }
-    // push stream: pop_swap[x + xi_q, q] = f*_q (Collisions.py:487-530)
+    // push stream: pop_swap[x + xi_q, q] = f*_q (Collisions.py:487-530);
+    // with walls, a destination that is a solid site sends f*_q back to x as
+    // the opposite population (Collisions.py:835-850)
     int p[3];
     geo.pos(g, p);
     for (int q = 0; q < Q; q++) {
       int e[3] = {S::xi[q][0], S::xi[q][1], S::xi[q][2]};
-      pop_swap[ord(geo.nb(p, e), q)] = fout[q];
+      int64_t dst = geo.nb(p, e);
+      int qd = q;
+      if (walls && q != 0 && walls[dst] == 0) {
+        dst = g;
+        qd = opposite<S>(q);
+      }
+      pop_swap[ord(dst, qd)] = fout[q];
     }
   }
   double* t = pop;
@@ -286,16 +314,21 @@ void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
 template <class S>
 void meta_moments(const Geo& geo, const EStencil& es, const Ord& ordp,
                   bool u_aos, double G, int psi_kind, const double* pop,
-                  double* n, double* u, double* psi) {
+                  double* n, double* u, double* psi, const uint8_t* walls) {
   constexpr int D = S::D, Q = S::Q;
   const int64_t V = geo.V;
+  // with walls: K_ComputeDensityPsiWallsMeta and
+  // K_ComputeVelocityAfterForceSCMPWallsMeta (MultiPhaseKernelsMeta.py:42-104,
+  // 1221-1365), fluid sites only
   for (int64_t g = 0; g < V; g++) {
+    if (walls && walls[g] != 1) continue;
     double ln = 0.;
     for (int q = 0; q < Q; q++) ln += pop[ordp(g, q)];
     psi[g] = psi_of(psi_kind, ln);
     n[g] = ln;
   }
   for (int64_t g = 0; g < V; g++) {
+    if (walls && walls[g] != 1) continue;
     double F[3];
     sc_force<D>(geo, es, psi, g, G, F);
     double lu[3] = {0., 0., 0.};
@@ -373,12 +406,13 @@ int cpuref_legacy_run(int xi_id, const int* L, int QE, const int* E,
 }
 
 // Runs nsteps of the Meta class MainLoop (mrt == 0) or MainLoopMRT (A, B, Gg
-// given). aos != 0 selects the CTypes 'cpu' ordering pop[q + g*Q].
+// given); with walls (V flags, 1 fluid / 0 solid; NULL for none) the step of
+// MainLoopWalls. aos != 0 selects the CTypes 'cpu' ordering pop[q + g*Q].
 int cpuref_meta_run(int xi_id, const int* L, int QE, const int* E,
                     const double* EW, double G, double omega, int psi_kind,
                     int aos, const double* A, const double* B, const double* Gg,
                     double* pop, double* pop_swap, double* n, double* psi,
-                    int nsteps) {
+                    int nsteps, const uint8_t* walls) {
   Geo geo;
   if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
   const int Q = xi_id == 0 ? 9 : 19;
@@ -388,8 +422,8 @@ int cpuref_meta_run(int xi_id, const int* L, int QE, const int* E,
   const MRTMats* mp = A ? &m : nullptr;
   double *a = pop, *b = pop_swap;
   for (int s = 0; s < nsteps; s++) {
-    if (xi_id == 0) meta_step<SD2Q9>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi);
-    else meta_step<SD3Q19>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi);
+    if (xi_id == 0) meta_step<SD2Q9>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi, walls);
+    else meta_step<SD3Q19>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi, walls);
   }
   if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
   return 0;
@@ -397,14 +431,17 @@ int cpuref_meta_run(int xi_id, const int* L, int QE, const int* E,
 
 int cpuref_meta_moments(int xi_id, const int* L, int QE, const int* E,
                         const double* EW, double G, int psi_kind, int aos,
-                        const double* pop, double* n, double* u, double* psi) {
+                        const double* pop, double* n, double* u, double* psi,
+                        const uint8_t* walls) {
   Geo geo;
   if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
   const int Q = xi_id == 0 ? 9 : 19;
   EStencil es{QE, E, EW};
   Ord ord{aos != 0, geo.V, Q};
-  if (xi_id == 0) meta_moments<SD2Q9>(geo, es, ord, aos != 0, G, psi_kind, pop, n, u, psi);
-  else meta_moments<SD3Q19>(geo, es, ord, aos != 0, G, psi_kind, pop, n, u, psi);
+  if (xi_id == 0)
+    meta_moments<SD2Q9>(geo, es, ord, aos != 0, G, psi_kind, pop, n, u, psi, walls);
+  else
+    meta_moments<SD3Q19>(geo, es, ord, aos != 0, G, psi_kind, pop, n, u, psi, walls);
   return 0;
 }
 

@@ -29,9 +29,10 @@ def lib():
                                         C.c_int]
         L.cpuref_meta_run.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
                                       C.c_double, C.c_int, C.c_int, C.c_void_p,
-                                      C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]
+                                      C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, C.c_int,
+                                      C.c_void_p]
         L.cpuref_meta_moments.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
-                                          C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+                                          C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p]
         L.cpuref_init_pop.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp]
         L.cpuref_init_radial.argtypes = [C.c_int, _ip, C.c_double, C.c_double,
                                          C.c_double, C.c_double, _dp, _dp]
@@ -100,10 +101,21 @@ def legacy_run(st, estencil, G, tau, psi='exp', nsteps=1):
     assert rc == 0
 
 
-def meta_run(st, estencil, G, tau, psi='exp', nsteps=1, mrt=None):
+def _walls_ptr(walls, V):
+    if walls is None:
+        return None, None
+    w = np.ascontiguousarray(walls, dtype=np.uint8).ravel()
+    assert w.size == V
+    return w, w.ctypes.data
+
+
+def meta_run(st, estencil, G, tau, psi='exp', nsteps=1, mrt=None, walls=None):
     """Meta class MainLoop / MainLoopMRT, nsteps
-    (idpy/LBM/MultiPhase.py:1015-1097, 1311-1402). mrt = (A, B, Gg)."""
+    (idpy/LBM/MultiPhase.py:1015-1097, 1311-1402). mrt = (A, B, Gg). walls
+    (V flags, 1 fluid / 0 solid, lexicographic): the step of MainLoopWalls
+    (MultiPhase.py:1226-1309); st.psi must then hold the solid sites' psi."""
     QE, Ei, Ew = _estencil(estencil)
+    wkeep, wptr = _walls_ptr(walls, st.V)
     ptrs = [None, None, None]
     keep = []
     if mrt is not None:
@@ -113,14 +125,17 @@ def meta_run(st, estencil, G, tau, psi='exp', nsteps=1, mrt=None):
             ptrs[i] = a.ctypes.data
     rc = lib().cpuref_meta_run(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, 1.0 / tau,
                                PSI_KIND[psi], int(st.aos), ptrs[0], ptrs[1], ptrs[2],
-                               st.pop, st.pop_swap, st.n, st.psi, nsteps)
+                               st.pop, st.pop_swap, st.n, st.psi, nsteps, wptr)
     assert rc == 0
 
 
-def meta_moments(st, estencil, G, psi='exp'):
+def meta_moments(st, estencil, G, psi='exp', walls=None):
     """ComputeMomentsMeta (idpy/LBM/MultiPhase.py:1565-1662): fills n, psi, u
-    with u = sum xi f / n + F / (2n); u in the same ordering as pop."""
+    with u = sum xi f / n + F / (2n); u in the same ordering as pop. With
+    walls only the fluid sites are written."""
     QE, Ei, Ew = _estencil(estencil)
+    wkeep, wptr = _walls_ptr(walls, st.V)
     rc = lib().cpuref_meta_moments(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G,
-                                   PSI_KIND[psi], int(st.aos), st.pop, st.n, st.u, st.psi)
+                                   PSI_KIND[psi], int(st.aos), st.pop, st.n, st.u, st.psi,
+                                   wptr)
     assert rc == 0

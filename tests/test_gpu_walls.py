@@ -1,0 +1,97 @@
+"""MainLoopWalls (idpy/LBM/MultiPhase.py:1226-1309) through the drop-in
+class on the HIP backend against the oracle: fluid sites only, half-way
+bounce-back at solid sites, the solid sites' psi set by GetWalls
+(MultiPhase.py:157-161). Bar: <= 1e-12 relative on the fluid sites."""
+import numpy as np
+import pytest
+import sympy as sp
+
+from helpers import droplet_field, rel_err
+
+pytestmark = pytest.mark.gpu
+
+G = -5.0 / 3.0
+
+
+def _geometry(dims, seed):
+    """Plates at y = 0 and y = L1 - 1, a solid block, wetting psi on solids."""
+    rng = np.random.default_rng(seed)
+    w = np.ones(tuple(reversed(dims)), dtype=np.uint8)
+    if len(dims) == 2:
+        w[0, :] = 0
+        w[-1, :] = 0
+        w[dims[1] // 3:dims[1] // 3 + 4, 5:11] = 0
+    else:
+        w[:, 0, :] = 0
+        w[:, -1, :] = 0
+        w[2:5, 4:8, 3:9] = 0
+    psi_w = np.where(w == 0, rng.uniform(0.3, 0.7, size=w.shape), 0.0)
+    return w, psi_w
+
+
+@pytest.mark.parametrize('xi,es,dims,nsteps', [('D2Q9', 'D2E4', (64, 37), 300),
+                                               ('D2Q9', 'D2E8', (48, 33), 120),
+                                               ('D3Q19', 'D3E4', (21, 18, 13), 60)])
+def test_mainloopwalls_matches_oracle(tenet, xi, es, dims, nsteps):
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    from oracle import cpuref
+    walls, psi_w = _geometry(dims, 5)
+    n0 = droplet_field(dims, seed=45)
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=dims, xi_stencil=XIStencils[xi],
+                             f_stencil=FStencils[es], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=G, tau=1.0)
+    sim.SetDensityField(n0)
+    sim.GetWalls(walls, psi_w)
+    sim.MainLoopWalls([0, nsteps // 2, nsteps])
+    st = cpuref.State(xi, dims)
+    cpuref.init_pop(st, n0.ravel(), np.zeros(st.V * st.D))
+    st.psi[:] = psi_w.ravel()
+    cpuref.meta_run(st, es, G, 1.0, nsteps=nsteps, walls=walls.ravel())
+    fl = walls.ravel() == 1
+    Q, V = st.Q, st.V
+    pop = sim.sims_idpy_memory['pop'].D2H().reshape(Q, V)
+    e_pop = rel_err(pop[:, fl].ravel(), st.pop.reshape(Q, V)[:, fl].ravel())
+    psi = sim.sims_idpy_memory['psi'].D2H()
+    e_psi = rel_err(psi[fl], st.psi[fl])
+    # the solid sites keep GetWalls' psi
+    assert np.array_equal(psi[~fl], psi_w.ravel()[~fl])
+    # ComputeMomentsMeta with walls: fluid-site n, psi and Guo-shifted u
+    sim.ComputeMomentsMeta()
+    cpuref.meta_moments(st, es, G, walls=walls.ravel())
+    n_d = sim.sims_idpy_memory['n'].D2H()
+    u_d = sim.sims_idpy_memory['u'].D2H().reshape(st.D, V)
+    e_n = rel_err(n_d[fl], st.n[fl])
+    e_u = rel_err(u_d[:, fl].ravel(), st.u.reshape(st.D, V)[:, fl].ravel())
+    m0 = float(n0.ravel()[fl].sum())
+    m1 = float(pop[:, fl].sum())
+    sim.End()
+    print('rel err pop', e_pop, 'psi', e_psi, 'n', e_n, 'u', e_u, 'mass drift',
+          abs(m1 - m0) / m0, flush=True)
+    assert e_pop <= 1e-12 and e_psi <= 1e-12 and e_n <= 1e-12
+    assert e_u <= 1e-11
+    assert abs(m1 - m0) <= 1e-13 * m0
+
+
+def test_mainloop_ignores_walls_and_moments_honour_them(tenet):
+    """MainLoop (no walls) after GetWalls steps the periodic lattice, as on
+    the reference, where only MainLoopWalls binds the walls array."""
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    from oracle import cpuref
+    dims = (40, 31)
+    walls, psi_w = _geometry(dims, 6)
+    n0 = droplet_field(dims, seed=46)
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=dims, xi_stencil=XIStencils['D2Q9'],
+                             f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=G, tau=1.0)
+    sim.SetDensityField(n0)
+    sim.GetWalls(walls, psi_w)
+    sim.MainLoop([0, 25])
+    st = cpuref.State('D2Q9', dims)
+    cpuref.init_pop(st, n0.ravel(), np.zeros(st.V * st.D))
+    cpuref.meta_run(st, 'D2E4', G, 1.0, nsteps=25)
+    assert rel_err(sim.sims_idpy_memory['pop'].D2H().ravel(), st.pop) <= 1e-12
+    sim.End()

@@ -1,0 +1,131 @@
+"""Pins of the oracle's walls step (MainLoopWalls, idpy/LBM/MultiPhase.py:
+1226-1309; no GPU): reduction to the periodic step, an exact bounce-back
+permutation, a stationary rest state, fluid mass conservation and an
+independent numpy gather restatement of the push-with-bounce-back rule."""
+import numpy as np
+import pytest
+
+from helpers import droplet_field, rel_err
+
+G = -5.0 / 3.0
+
+
+def _channel(dims, seed):
+    """Solid plates at y = 0 and y = L1 - 1 plus a solid block; 1 fluid."""
+    rng = np.random.default_rng(seed)
+    w = np.ones(tuple(reversed(dims)), dtype=np.uint8)
+    if len(dims) == 2:
+        w[0, :] = 0
+        w[-1, :] = 0
+        w[3:6, 4:7] = 0
+    else:
+        w[:, 0, :] = 0
+        w[:, -1, :] = 0
+        w[2:4, 3:5, 2:5] = 0
+    psi_w = rng.uniform(0.2, 0.8, size=w.shape)
+    return w, psi_w
+
+
+def _state(xi, dims, n0):
+    from oracle import cpuref
+    st = cpuref.State(xi, dims)
+    cpuref.init_pop(st, n0.ravel(), np.zeros(st.V * st.D))
+    return st
+
+
+@pytest.mark.parametrize('xi,es,dims', [('D2Q9', 'D2E4', (23, 17)),
+                                        ('D3Q19', 'D3E4', (9, 8, 7))])
+def test_all_fluid_walls_equal_the_periodic_step(xi, es, dims):
+    from oracle import cpuref
+    n0 = droplet_field(dims, seed=41)
+    a, b = _state(xi, dims, n0), _state(xi, dims, n0)
+    cpuref.meta_run(a, es, G, 1.0, nsteps=7)
+    cpuref.meta_run(b, es, G, 1.0, nsteps=7, walls=np.ones(b.V, dtype=np.uint8))
+    assert np.array_equal(a.pop, b.pop)
+
+
+@pytest.mark.parametrize('xi,dims', [('D2Q9', (9, 7)), ('D3Q19', (7, 6, 5))])
+def test_bounce_back_is_an_exact_reflection(xi, dims):
+    """Pure streaming (G = 0, omega ~ 0): a spike moving into a solid site
+    comes back at its own site in the opposite direction after one step; a
+    spike moving along the wall streams normally."""
+    from oracle import cpuref
+    from oracle.stencils import XI, XI_W
+    xis = XI[xi]
+    Q, D, V = len(xis), len(dims), int(np.prod(dims))
+    walls = np.ones(tuple(reversed(dims)), dtype=np.uint8)
+    if D == 2:
+        walls[0, :] = 0
+    else:
+        walls[:, 0, :] = 0
+    flat = walls.ravel()
+    st = cpuref.State(xi, dims)
+    f = np.array([float(w) for w in XI_W[xi]])[:, None] * np.ones(V)
+    site = 3 + dims[0] * 1  # (x=3, y=1): fluid, next to the plate at y = 0
+    down = next(q for q, v in enumerate(xis) if v[1] == -1 and v[0] == 1 and (D == 2 or v[2] == 0))
+    up = next(q for q, v in enumerate(xis) if list(v) == [-c for c in xis[down]])
+    along = next(q for q, v in enumerate(xis) if v[0] == 1 and v[1] == 0 and (D == 2 or v[2] == 0))
+    f[down, site] += 0.5
+    f[along, site] += 0.25
+    st.pop[:] = f.ravel()
+    st.psi[:] = 0.0
+    cpuref.meta_run(st, 'D2E4' if D == 2 else 'D3E4', 0.0, 1e300, nsteps=1, walls=flat)
+    out = st.pop.reshape(Q, V)
+    base = np.array([float(w) for w in XI_W[xi]])
+    assert out[up, site] == base[up] + 0.5
+    assert out[along, site + 1] == base[along] + 0.25
+    fl = flat == 1
+    assert np.isclose(out[:, fl].sum(), f[:, fl].sum(), rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize('xi,es,dims', [('D2Q9', 'D2E4', (16, 12)),
+                                        ('D3Q19', 'D3E4', (8, 8, 6))])
+def test_rest_state_with_matching_wall_psi_is_stationary(xi, es, dims):
+    """Uniform density, psi_w = psi(n): no force, and bounce-back maps the
+    rest equilibrium onto itself (w_q = w_opp(q))."""
+    from oracle import cpuref
+    walls, _ = _channel(dims, 1)
+    n0 = np.full(walls.shape, 1.3)
+    st = _state(xi, dims, n0)
+    st.psi[:] = 1.0 - np.exp(-1.3)
+    f0 = st.pop.copy().reshape(st.Q, st.V)
+    cpuref.meta_run(st, es, G, 1.0, nsteps=40, walls=walls.ravel())
+    fl = walls.ravel() == 1
+    assert np.abs(st.pop.reshape(st.Q, st.V)[:, fl] - f0[:, fl]).max() < 1e-14
+
+
+@pytest.mark.parametrize('xi,es,dims', [('D2Q9', 'D2E4', (23, 17)),
+                                        ('D3Q19', 'D3E4', (9, 8, 7))])
+def test_walls_conserve_fluid_mass(xi, es, dims):
+    from oracle import cpuref
+    walls, psi_w = _channel(dims, 2)
+    n0 = droplet_field(dims, seed=43)
+    st = _state(xi, dims, n0)
+    st.psi[:] = psi_w.ravel()
+    fl = walls.ravel() == 1
+    m0 = st.pop.reshape(st.Q, st.V)[:, fl].sum()
+    cpuref.meta_run(st, es, G, 1.0, nsteps=150, walls=walls.ravel())
+    m1 = st.pop.reshape(st.Q, st.V)[:, fl].sum()
+    assert abs(m1 - m0) < 1e-13 * m0
+
+
+@pytest.mark.parametrize('xi,es,dims', [('D2Q9', 'D2E4', (23, 17)), ('D2Q9', 'D2E8', (19, 16)),
+                                        ('D3Q19', 'D3E4', (9, 8, 7))])
+def test_walls_oracle_matches_numpy_gather(xi, es, dims):
+    from oracle import cpuref, np_ref
+    walls, psi_w = _channel(dims, 3)
+    n0 = droplet_field(dims, seed=44)
+    st = _state(xi, dims, n0)
+    st.psi[:] = psi_w.ravel()
+    shape = (st.Q,) + walls.shape
+    f = st.pop.reshape(shape).copy()
+    old = st.pop_swap.reshape(shape).copy()
+    for k in range(4):
+        f_new, n, psi = np_ref.step_walls(f, xi, es, G, 1.0, walls, psi_w, old)
+        old, f = f, f_new
+    cpuref.meta_run(st, es, G, 1.0, nsteps=4, walls=walls.ravel())
+    fl = walls == 1
+    got = st.pop.reshape(shape)[:, fl]
+    assert rel_err(got.ravel(), f[:, fl].ravel()) < 1e-13
+    assert rel_err(st.n.reshape(walls.shape)[fl], n[fl]) < 1e-13
+    assert np.array_equal(st.psi.reshape(walls.shape)[~fl], psi_w[~fl])
