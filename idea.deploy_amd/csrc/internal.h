@@ -44,6 +44,7 @@ struct idmx_sc_s {
   int tx = 0, ty = 0, H = 0, hist = 1;
   int coll = 0;  // 0 BGK, 1 dense MRT, 2 diagonal MRT
   const uint8_t* walls = nullptr;  // device flags, 1 fluid / 0 solid (borrowed)
+  double bforce[3] = {0.0, 0.0, 0.0};  // body force added to the Shan-Chen force
 };
 
 // Per-(velocity set, E-stencil, precision, collision) entry points.

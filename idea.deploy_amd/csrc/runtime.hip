@@ -1,6 +1,7 @@
 // runtime.hip -- context, device memory, reductions and the Shan-Chen
 // simulation object of libidmx.so (C ABI declared in include/idmx.h).
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -466,6 +467,16 @@ int idmx_sc_set_launch(idmx_sc_t s, int tx, int ty, int H, int hist) {
 int idmx_sc_set_walls(idmx_sc_t s, const void* walls) {
   CHECK_ARG(s, "idmx_sc_set_walls: null simulation");
   s->walls = (const uint8_t*)walls;
+  return IDMX_OK;
+}
+
+int idmx_sc_set_body_force(idmx_sc_t s, double fx, double fy, double fz) {
+  CHECK_ARG(s, "idmx_sc_set_body_force: null simulation");
+  CHECK_ARG(std::isfinite(fx) && std::isfinite(fy) && std::isfinite(fz),
+            "idmx_sc_set_body_force: non-finite force");
+  s->bforce[0] = fx;
+  s->bforce[1] = fy;
+  s->bforce[2] = s->D == 3 ? fz : 0.0;
   return IDMX_OK;
 }
 

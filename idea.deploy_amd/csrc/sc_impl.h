@@ -145,10 +145,12 @@ struct Impl {
                        s->ctx->stream, pop, s->geo, phys(s), n, psi, s->walls);
     if (wf)
       hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, true>), dim3(grid), dim3(256), 0,
-                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u, s->walls);
+                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u, s->walls,
+                         s->bforce[0], s->bforce[1], s->bforce[2]);
     else
       hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, false>), dim3(grid), dim3(256), 0,
-                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u, s->walls);
+                         s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u, s->walls,
+                         s->bforce[0], s->bforce[1], s->bforce[2]);
     return hipGetLastError();
   }
 
@@ -157,10 +159,12 @@ struct Impl {
     const bool wf = (flags & IDMX_RUN_WRITE_FIELDS) != 0;
     // with walls the step is the two-pass one (density and psi of the fluid
     // sites into n / psi, where psi also holds the solid sites' values)
-    const bool twop = (flags & IDMX_RUN_TWO_PASS) != 0 || s->walls;
+    const bool body = s->bforce[0] != 0.0 || s->bforce[1] != 0.0 || s->bforce[2] != 0.0;
+    const bool twop = (flags & IDMX_RUN_TWO_PASS) != 0 || s->walls || body;
     if (twop && (!n || !psi)) {
-      set_error(s->walls ? "idmx_sc_run: a simulation with walls needs n and psi buffers"
-                         : "idmx_sc_run: the two-pass step needs n and psi buffers");
+      set_error(s->walls || body
+                    ? "idmx_sc_run: walls or a body force need the n and psi buffers"
+                    : "idmx_sc_run: the two-pass step needs n and psi buffers");
       return IDMX_E_ARG;
     }
     if (wf && (!n || !psi || !u)) {
@@ -217,11 +221,11 @@ struct Impl {
     if (u)
       hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, true>), dim3(grid), dim3(256), 0,
                          s->ctx->stream, (const T*)pop, (T*)swap, n, psi, s->geo, phys(s), u,
-                         s->walls);
+                         s->walls, s->bforce[0], s->bforce[1], s->bforce[2]);
     else
       hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, false>), dim3(grid), dim3(256), 0,
                          s->ctx->stream, (const T*)pop, (T*)swap, n, psi, s->geo, phys(s),
-                         nullptr, s->walls);
+                         nullptr, s->walls, s->bforce[0], s->bforce[1], s->bforce[2]);
     IDMX_HIP(hipGetLastError());
     return IDMX_OK;
   }
@@ -232,7 +236,7 @@ struct Impl {
                        s->ctx->stream, (const T*)pop, s->geo, phys(s), n, psi, s->walls);
     hipLaunchKernelGGL((k_velocity<S, ES, T, SHIFT>), dim3(grid), dim3(256), 0,
                        s->ctx->stream, (const T*)pop, n, psi, s->geo, s->desc.SC_G, u,
-                       s->walls);
+                       s->walls, s->bforce[0], s->bforce[1], s->bforce[2]);
     IDMX_HIP(hipGetLastError());
     return IDMX_OK;
   }
@@ -244,7 +248,8 @@ struct Impl {
                        s->ctx->stream, (const T*)pop, s->geo, phys(s), n, nullptr, nullptr);
     // G = 0: no force term; n stands in for the (unused) psi argument
     hipLaunchKernelGGL((k_velocity<S, ES, T, SHIFT>), dim3(grid), dim3(256), 0,
-                       s->ctx->stream, (const T*)pop, n, n, s->geo, 0.0, u, nullptr);
+                       s->ctx->stream, (const T*)pop, n, n, s->geo, 0.0, u, nullptr, 0.0, 0.0,
+                       0.0);
     IDMX_HIP(hipGetLastError());
     return IDMX_OK;
   }

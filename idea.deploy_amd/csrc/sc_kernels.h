@@ -108,11 +108,14 @@ __device__ constexpr bool mpv_row_nz(int k) {
 // --------------------------------------------------------------------------
 // Collision of one site. f holds the (shifted when SHIFT) populations, dn is
 // sum_q f_q of those values, (gx, gy, gz) the isotropic gradient of psi and
-// lpsi = psi(x). Returns f* in fo and the Guo-shifted velocity in u.
-template <class S, class T, bool SHIFT, int COLL>
+// lpsi = psi(x). Returns f* in fo and the Guo-shifted velocity in u. With BF
+// a body force (bx, by, bz) is added to the Shan-Chen force (the gravity of
+// K_Collision_ShanChenGuoMultiPhaseWallsGravity, F[1] += g,
+// idpy/LBM/MultiPhaseKernels.py:474-560).
+template <class S, class T, bool SHIFT, int COLL, bool BF = false>
 __device__ inline void collide(const T (&f)[S::Q], T dn, T gx, T gy, T gz,
                                T lpsi, const Phys<T>& P, T (&fo)[S::Q],
-                               T (&u)[3]) {
+                               T (&u)[3], T bx = T(0), T by = T(0), T bz = T(0)) {
   constexpr int Q = S::Q;
   T jx = 0, jy = 0, jz = 0;
 #pragma unroll
@@ -126,7 +129,12 @@ __device__ inline void collide(const T (&f)[S::Q], T dn, T gx, T gy, T gz,
   }
   const T n = SHIFT ? P.rho0 + dn : dn;
   const T mGpsi = -P.G * lpsi;
-  const T Fx = mGpsi * gx, Fy = mGpsi * gy, Fz = S::D == 3 ? mGpsi * gz : T(0);
+  T Fx = mGpsi * gx, Fy = mGpsi * gy, Fz = S::D == 3 ? mGpsi * gz : T(0);
+  if constexpr (BF) {
+    Fx += bx;
+    Fy += by;
+    if (S::D == 3) Fz += bz;
+  }
   const T inv_n = T(1) / n;
   const T ux = (jx + T(0.5) * Fx) * inv_n;
   const T uy = (jy + T(0.5) * Fy) * inv_n;
@@ -982,12 +990,14 @@ __device__ inline void site_xyz(const Geom& geo, int64_t g, int& x, int& y, int&
 // K_ForceCollideStreamSCMPMeta / _MRTMeta (MultiPhaseKernelsMeta.py:169-345,
 // 891-1075): force from the psi array, n from the n array, push stream.
 // With walls, K_ForceCollideStreamWallsSCMPMeta (:524-716): fluid sites only,
-// bounce-back at solid sites.
+// bounce-back at solid sites. (bx, by, bz) is a body force added to the
+// Shan-Chen force (zero unless set).
 template <class S, class ES, class T, bool SHIFT, int COLL, bool WF>
 __global__ void __launch_bounds__(256)
 k_fcs(const T* __restrict__ pop, T* __restrict__ pop_swap,
       const double* __restrict__ n_in, const double* __restrict__ psi_in,
-      Geom geo, Phys<T> P, double* __restrict__ u_out, const uint8_t* __restrict__ walls) {
+      Geom geo, Phys<T> P, double* __restrict__ u_out, const uint8_t* __restrict__ walls,
+      double bx, double by, double bz) {
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
        g += (int64_t)gridDim.x * blockDim.x) {
     if (walls && walls[g] != 1) continue;
@@ -1001,7 +1011,8 @@ k_fcs(const T* __restrict__ pop, T* __restrict__ pop_swap,
     const T ln = (T)n_in[g];
     const T dn = SHIFT ? ln - P.rho0 : ln;
     T fo[S::Q], u[3];
-    collide<S, T, SHIFT, COLL>(f, dn, gx, gy, gz, (T)psi_in[g], P, fo, u);
+    collide<S, T, SHIFT, COLL, true>(f, dn, gx, gy, gz, (T)psi_in[g], P, fo, u, (T)bx, (T)by,
+                                     (T)bz);
     if (walls) push_store_walls<S, T>(pop_swap, geo, walls, g, x, y, z, fo);
     else push_store<S, T>(pop_swap, geo, x, y, z, fo);
     if (WF) {
@@ -1019,7 +1030,8 @@ template <class S, class ES, class T, bool SHIFT>
 __global__ void __launch_bounds__(256)
 k_velocity(const T* __restrict__ pop, const double* __restrict__ n_in,
            const double* __restrict__ psi_in, Geom geo, double G,
-           double* __restrict__ u_out, const uint8_t* __restrict__ walls) {
+           double* __restrict__ u_out, const uint8_t* __restrict__ walls, double bx, double by,
+           double bz) {
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
        g += (int64_t)gridDim.x * blockDim.x) {
     if (walls && walls[g] != 1) continue;
@@ -1037,7 +1049,7 @@ k_velocity(const T* __restrict__ pop, const double* __restrict__ n_in,
     }
     const double ln = n_in[g];
     const double mGpsi = -G * psi_in[g];
-    const double F[3] = {mGpsi * gx, mGpsi * gy, mGpsi * gz};
+    const double F[3] = {mGpsi * gx + bx, mGpsi * gy + by, mGpsi * gz + bz};
     for (int d = 0; d < S::D; d++) u_out[g + d * geo.V] = j[d] / ln + 0.5 * F[d] / ln;
   }
 }

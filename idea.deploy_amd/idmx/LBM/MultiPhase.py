@@ -94,6 +94,34 @@ class ShanChenMultiPhase(_ShanChenCommon):
         finally:
             self.engine.set_walls(None)
 
+    def MainLoopSimpleWalls(self, time_steps, convergence_functions=[], profiling=False):
+        """MainLoopSimpleWalls (MultiPhase.py:351-465): the legacy-kernel walls
+        loop (K_ComputeMomentsWalls, K_ComputePsiWalls,
+        K_Collision_ShanChenGuoMultiPhaseWalls, K_StreamPeriodic,
+        K_HalfWayBounceBack). Fluid sites follow the same update as
+        MainLoopWalls; u (Guo-shifted), n and psi are written as by the
+        legacy kernels. The solid sites' populations are not defined (the
+        reference leaves them order-dependent)."""
+        return self.MainLoopSimpleWallsGravity(time_steps, convergence_functions, g=0,
+                                               profiling=profiling)
+
+    def MainLoopSimpleWallsGravity(self, time_steps, convergence_functions=[], g=0,
+                                   profiling=False):
+        """MainLoopSimpleWallsGravity (MultiPhase.py:888-1010): as
+        MainLoopSimpleWalls with the gravity g added to the y component of the
+        force (F[1] += g, MultiPhaseKernels.py:474-560)."""
+        m = self.sims_idpy_memory
+        if 'walls' not in m:
+            raise Exception("Missing 'walls': call GetWalls first")
+        self.engine.set_walls(m['walls'])
+        self.engine.set_body_force(0.0, float(g), 0.0)
+        try:
+            return self._run_loop(time_steps, convergence_functions, profiling, self.engine,
+                                  m['u'])
+        finally:
+            self.engine.set_walls(None)
+            self.engine.set_body_force(0.0, 0.0, 0.0)
+
     def GetWalls(self, walls, psi_w):
         """psi_w (V values, the psi of the solid sites; the fluid sites' are
         recomputed every step) and the walls flags (1 fluid, 0 solid, shape

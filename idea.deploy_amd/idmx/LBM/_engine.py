@@ -122,6 +122,11 @@ class SCEngine:
         While set, run / moments step as MainLoopWalls."""
         _lib.call('idmx_sc_set_walls', self.handle, walls.ptr if walls is not None else None)
 
+    def set_body_force(self, fx=0.0, fy=0.0, fz=0.0):
+        """Body force added to the Shan-Chen force (gravity of
+        MainLoopSimpleWallsGravity); zero restores the plain step."""
+        _lib.call('idmx_sc_set_body_force', self.handle, float(fx), float(fy), float(fz))
+
     def launch_ok(self, tx, ty, H, hist=1):
         """Whether set_launch accepts this shape (variant 9 needs its LDS
         staging buffer to fit one CU); the current launch is kept."""

@@ -65,6 +65,7 @@ _SIGS = [
     ('idmx_sc_pop_bytes', C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     ('idmx_sc_set_launch', C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     ('idmx_sc_set_walls', C.c_int, [C.c_void_p, C.c_void_p]),
+    ('idmx_sc_set_body_force', C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     ('idmx_sc_get_launch', C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ('idmx_sc_init_pop', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -233,6 +233,14 @@ int idmx_sc_set_launch(idmx_sc_t sim, int tx, int ty, int H, int hist);
  * MultiPhase.py:157-161). idmx_sc_density_psi, idmx_sc_force_collide_stream
  * and idmx_sc_moments honour the walls the same way. */
 int idmx_sc_set_walls(idmx_sc_t sim, const void* walls);
+
+/* Body force (fx, fy, fz; fz ignored in 2D) added to the Shan-Chen force at
+ * every updated site, in the velocity shift and the Guo term: the gravity of
+ * MainLoopSimpleWallsGravity (F[1] += g,
+ * idpy/LBM/MultiPhaseKernels.py:474-560, idpy/LBM/MultiPhase.py:888-1010).
+ * Zero by default; while non-zero, idmx_sc_run uses the two-pass step (n and
+ * psi buffers required) and idmx_sc_moments includes it in u. */
+int idmx_sc_set_body_force(idmx_sc_t sim, double fx, double fy, double fz);
 int idmx_sc_get_launch(idmx_sc_t sim, int* tx, int* ty, int* H, int* hist);
 
 /* Milliseconds of the last idmx_sc_run, measured with HIP events recorded

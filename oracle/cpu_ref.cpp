@@ -29,6 +29,10 @@
 //     (:524-716), whose push stream reflects a population aimed at a solid
 //     site (walls == 0) back into the opposite direction at its source site
 //     (SRTCollisionPlusGuoPushStreamWallsCode, idpy/LBM/Collisions.py:835-850);
+//   * MainLoopSimpleWalls / MainLoopSimpleWallsGravity (idpy/LBM/MultiPhase.py:
+//     351-465, 888-1010): the legacy kernels on the flagged sites, pull
+//     streaming, then K_HalfWayBounceBack (idpy/LBM/LBMKernels.py:151-184)
+//     and a body force F[1] += g;
 //   * ComputeMomentsMeta (idpy/LBM/MultiPhase.py:1565-1662,
 //     K_ComputeVelocityAfterForceSCMPMeta MultiPhaseKernelsMeta.py:1077-1219)
 //   * InitPopulations (K_InitPopulations idpy/LBM/LBM.py:1700-1743)
@@ -219,6 +223,86 @@ void legacy_step(const Geo& geo, const EStencil& es, double G, double omega,
   pop_swap = t;
 }
 
+// ------------------------------- legacy-kernel wall loops (SoA, pull stream)
+// MainLoopSimpleWalls / MainLoopSimpleWallsGravity (idpy/LBM/MultiPhase.py:
+// 351-465, 888-1010): K_ComputeMomentsWalls (LBMKernels.py:220-253, walls ==
+// 1), K_ComputePsiWalls (MultiPhaseKernels.py:583-603, walls != 0),
+// K_Collision_ShanChenGuoMultiPhaseWalls[Gravity] (MultiPhaseKernels.py:
+// 389-560, F[1] += g), K_StreamPeriodic (LBMKernels.py:117-149, every site),
+// K_HalfWayBounceBack on the streamed populations (LBMKernels.py:151-184,
+// walls == 0, in the serial site order of the CTypes loop), M_SwapPop.
+template <class S>
+int opposite(int q);
+
+template <class S>
+void legacy_walls_step(const Geo& geo, const EStencil& es, double G, double omega,
+                       int psi_kind, const uint8_t* walls, double gy, double*& pop,
+                       double*& pop_swap, double* n, double* u, double* psi) {
+  constexpr int D = S::D, Q = S::Q;
+  const int64_t V = geo.V;
+  for (int64_t g = 0; g < V; g++) {
+    if (walls[g] != 1) continue;
+    double lu[3] = {0., 0., 0.};
+    double ln = 0.;
+    for (int q = 0; q < Q; q++) {
+      const double lpop = pop[g + q * V];
+      ln += lpop;
+      for (int d = 0; d < D; d++) lu[d] += lpop * S::xi[q][d];
+    }
+    n[g] = ln;
+    for (int d = 0; d < D; d++) u[g + d * V] = lu[d] / ln;
+  }
+  for (int64_t g = 0; g < V; g++)
+    if (walls[g]) psi[g] = psi_of(psi_kind, n[g]);
+  for (int64_t g = 0; g < V; g++) {
+    if (!walls[g]) continue;
+    double F[3];
+    sc_force<D>(geo, es, psi, g, G, F);
+    F[1] += gy;
+    const double ln = n[g];
+    double lu[3];
+    for (int d = 0; d < D; d++) {
+      lu[d] = u[g + V * d] + 0.5 * F[d] / ln;
+      u[g + V * d] = lu[d];
+    }
+    double u_dot_u = 0., F_dot_u = 0.;
+    for (int d = 0; d < D; d++) {
+      u_dot_u += lu[d] * lu[d];
+      F_dot_u += F[d] * lu[d];
+    }
+    for (int q = 0; q < Q; q++) {
+      double leq, lguo;
+      eq_guo<S>(q, ln, lu, F, u_dot_u, F_dot_u, &leq, &lguo);
+      pop[g + q * V] = pop[g + q * V] * (1. - omega) + leq * omega +
+                       (1. - 0.5 * omega) * lguo;
+    }
+  }
+  for (int64_t g = 0; g < V; g++) {
+    int p[3];
+    geo.pos(g, p);
+    pop_swap[g] = pop[g];
+    for (int q = 1; q < Q; q++) {
+      int e[3] = {-S::xi[q][0], -S::xi[q][1], -S::xi[q][2]};
+      pop_swap[g + q * V] = pop[geo.nb(p, e) + q * V];
+    }
+  }
+  for (int64_t g = 0; g < V; g++) {
+    if (walls[g] != 0) continue;
+    int p[3];
+    geo.pos(g, p);
+    pop_swap[g] = 0.;
+    for (int q = 1; q < Q; q++) {
+      const int qd = opposite<S>(q);
+      int e[3] = {S::xi[qd][0], S::xi[qd][1], S::xi[qd][2]};
+      pop_swap[geo.nb(p, e) + qd * V] = pop_swap[g + q * V];
+      pop_swap[g + q * V] = 0.;
+    }
+  }
+  double* t = pop;
+  pop = pop_swap;
+  pop_swap = t;
+}
+
 // ------------------------------------------- Meta class, BGK or MRT (push)
 struct MRTMats {
   const double* A;   // M^-1 (1-S) M     (Q x Q, row-major)
@@ -399,6 +483,28 @@ int cpuref_legacy_run(int xi_id, const int* L, int QE, const int* E,
   for (int s = 0; s < nsteps; s++) {
     if (xi_id == 0) legacy_step<SD2Q9>(geo, es, G, omega, psi_kind, a, b, n, u, psi);
     else legacy_step<SD3Q19>(geo, es, G, omega, psi_kind, a, b, n, u, psi);
+  }
+  const int Q = xi_id == 0 ? 9 : 19;
+  if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
+  return 0;
+}
+
+// Runs nsteps of MainLoopSimpleWalls (gy == 0) or MainLoopSimpleWallsGravity
+// (SoA); walls is V flags, 1 fluid / 0 solid. pop holds the state after the
+// call; n, u, psi hold the values of the last step.
+int cpuref_legacy_walls_run(int xi_id, const int* L, int QE, const int* E,
+                            const double* EW, double G, double omega, int psi_kind,
+                            const uint8_t* walls, double gy, double* pop, double* pop_swap,
+                            double* n, double* u, double* psi, int nsteps) {
+  Geo geo;
+  if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
+  EStencil es{QE, E, EW};
+  double *a = pop, *b = pop_swap;
+  for (int s = 0; s < nsteps; s++) {
+    if (xi_id == 0)
+      legacy_walls_step<SD2Q9>(geo, es, G, omega, psi_kind, walls, gy, a, b, n, u, psi);
+    else
+      legacy_walls_step<SD3Q19>(geo, es, G, omega, psi_kind, walls, gy, a, b, n, u, psi);
   }
   const int Q = xi_id == 0 ? 9 : 19;
   if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);

@@ -33,6 +33,9 @@ def lib():
                                       C.c_void_p]
         L.cpuref_meta_moments.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
                                           C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p]
+        L.cpuref_legacy_walls_run.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
+                                              C.c_double, C.c_int, C.c_void_p, C.c_double,
+                                              _dp, _dp, _dp, _dp, _dp, C.c_int]
         L.cpuref_init_pop.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp]
         L.cpuref_init_radial.argtypes = [C.c_int, _ip, C.c_double, C.c_double,
                                          C.c_double, C.c_double, _dp, _dp]
@@ -98,6 +101,19 @@ def legacy_run(st, estencil, G, tau, psi='exp', nsteps=1):
     rc = lib().cpuref_legacy_run(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, 1.0 / tau,
                                  PSI_KIND[psi], st.pop, st.pop_swap, st.n, st.u,
                                  st.psi, nsteps)
+    assert rc == 0
+
+
+def legacy_walls_run(st, estencil, G, tau, walls, g=0.0, psi='exp', nsteps=1):
+    """MainLoopSimpleWalls (g = 0) / MainLoopSimpleWallsGravity
+    (idpy/LBM/MultiPhase.py:351-465, 888-1010), nsteps; SoA; st.psi must hold
+    the solid sites' psi (GetWalls)."""
+    assert not st.aos
+    QE, Ei, Ew = _estencil(estencil)
+    wkeep, wptr = _walls_ptr(walls, st.V)
+    rc = lib().cpuref_legacy_walls_run(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, 1.0 / tau,
+                                       PSI_KIND[psi], wptr, float(g), st.pop, st.pop_swap,
+                                       st.n, st.u, st.psi, nsteps)
     assert rc == 0
 
 

@@ -76,14 +76,14 @@ def step(f, xi, estencil, G, tau, psi_kind='exp', mrt=None):
     return fn, n, psi, u
 
 
-def step_walls(f, xi, estencil, G, tau, walls, psi_solid, f_old, psi_kind='exp'):
+def step_walls(f, xi, estencil, G, tau, walls, psi_solid, f_old, psi_kind='exp', gy=0.0):
     """One step of MainLoopWalls (idpy/LBM/MultiPhase.py:1226-1309), written
     as a gather: population p of fluid site x comes from x - c_p when that
     site is fluid (push), or is its own f*_opp(p) when x - c_p is solid
     (half-way bounce-back, idpy/LBM/Collisions.py:835-850). walls: 1 fluid /
     0 solid, shape flip(dims); psi_solid: psi of the solid sites; f_old: the
-    destination buffer's previous contents (kept where nothing arrives).
-    Returns (f_new, n, psi)."""
+    destination buffer's previous contents (kept where nothing arrives); gy:
+    body force along y added to the Shan-Chen force. Returns (f_new, n, psi)."""
     omega = 1.0 / tau
     xis = XI[xi]
     D = len(xis[0])
@@ -92,6 +92,7 @@ def step_walls(f, xi, estencil, G, tau, walls, psi_solid, f_old, psi_kind='exp')
     n = np.where(fluid, f.sum(axis=0), 1.0)
     psi = np.where(fluid, psi_fn(n, psi_kind), psi_solid)
     F = sc_force(psi, estencil, G)
+    F[1] = F[1] + gy
     u = [sum(xis[q][d] * f[q] for q in range(len(xis))) / n + 0.5 * F[d] / n for d in range(D)]
     feq, fg = feq_guo(xi, n, u, F)
     fs = (1 - omega) * f + omega * feq + (1 - 0.5 * omega) * fg

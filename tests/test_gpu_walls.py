@@ -95,3 +95,45 @@ def test_mainloop_ignores_walls_and_moments_honour_them(tenet):
     cpuref.meta_run(st, 'D2E4', G, 1.0, nsteps=25)
     assert rel_err(sim.sims_idpy_memory['pop'].D2H().ravel(), st.pop) <= 1e-12
     sim.End()
+
+
+@pytest.mark.parametrize('xi,es,dims,g,nsteps', [('D2Q9', 'D2E4', (64, 37), 2e-5, 250),
+                                                 ('D2Q9', 'D2E4', (40, 29), 0.0, 80),
+                                                 ('D3Q19', 'D3E4', (21, 18, 13), 1e-5, 50)])
+def test_simple_walls_gravity_loop_matches_oracle(tenet, xi, es, dims, g, nsteps):
+    """MainLoopSimpleWallsGravity / MainLoopSimpleWalls against the oracle's
+    legacy-kernel restatement (pull stream + K_HalfWayBounceBack, F[1] += g):
+    fluid-site populations, n, psi and the Guo-shifted u the loop writes."""
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    from oracle import cpuref
+    walls, psi_w = _geometry(dims, 7)
+    n0 = droplet_field(dims, seed=47)
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=dims, xi_stencil=XIStencils[xi],
+                             f_stencil=FStencils[es], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=G, tau=1.0)
+    sim.SetDensityField(n0)
+    sim.GetWalls(walls, psi_w)
+    if g:
+        sim.MainLoopSimpleWallsGravity([0, nsteps // 3, nsteps], g=g)
+    else:
+        sim.MainLoopSimpleWalls([0, nsteps])
+    st = cpuref.State(xi, dims)
+    cpuref.init_pop(st, n0.ravel(), np.zeros(st.V * st.D))
+    st.psi[:] = psi_w.ravel()
+    cpuref.legacy_walls_run(st, es, G, 1.0, walls.ravel(), g, nsteps=nsteps)
+    fl = walls.ravel() == 1
+    Q, V, D = st.Q, st.V, st.D
+    m = sim.sims_idpy_memory
+    pop = m['pop'].D2H().reshape(Q, V)
+    e_pop = rel_err(pop[:, fl].ravel(), st.pop.reshape(Q, V)[:, fl].ravel())
+    e_n = rel_err(m['n'].D2H()[fl], st.n[fl])
+    e_psi = rel_err(m['psi'].D2H()[fl], st.psi[fl])
+    e_u = rel_err(m['u'].D2H().reshape(D, V)[:, fl].ravel(),
+                  st.u.reshape(D, V)[:, fl].ravel())
+    # the body force is cleared after the loop: MainLoop is the plain step
+    sim.End()
+    print('rel err pop', e_pop, 'n', e_n, 'psi', e_psi, 'u', e_u, flush=True)
+    assert e_pop <= 1e-12 and e_n <= 1e-12 and e_psi <= 1e-12
+    assert e_u <= 1e-11

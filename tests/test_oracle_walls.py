@@ -129,3 +129,66 @@ def test_walls_oracle_matches_numpy_gather(xi, es, dims):
     assert rel_err(got.ravel(), f[:, fl].ravel()) < 1e-13
     assert rel_err(st.n.reshape(walls.shape)[fl], n[fl]) < 1e-13
     assert np.array_equal(st.psi.reshape(walls.shape)[~fl], psi_w[~fl])
+
+
+# ------------------------------------- legacy-kernel wall loops with gravity
+@pytest.mark.parametrize('xi,es,dims', [('D2Q9', 'D2E4', (23, 17)),
+                                        ('D3Q19', 'D3E4', (9, 8, 7))])
+def test_legacy_walls_all_fluid_no_gravity_equal_legacy_loop(xi, es, dims):
+    from oracle import cpuref
+    n0 = droplet_field(dims, seed=51)
+    a, b = _state(xi, dims, n0), _state(xi, dims, n0)
+    cpuref.legacy_run(a, es, G, 1.0, nsteps=6)
+    cpuref.legacy_walls_run(b, es, G, 1.0, np.ones(b.V, dtype=np.uint8), 0.0, nsteps=6)
+    assert np.array_equal(a.pop, b.pop)
+    assert np.array_equal(a.u, b.u)
+
+
+@pytest.mark.parametrize('xi,dims,tau', [('D2Q9', (12, 10), 1.0), ('D2Q9', (12, 10), 0.7),
+                                         ('D3Q19', (6, 5, 4), 0.8)])
+def test_gravity_adds_exactly_g_momentum_per_site_and_step(xi, dims, tau):
+    """G = 0, periodic, uniform density at rest: with Guo forcing the total
+    momentum grows by the force exactly, sum_q xi f* = J + F (for any tau),
+    so after N steps J_y = N V g and J_x = 0."""
+    from oracle import cpuref
+    from oracle.stencils import XI
+    st = _state(xi, dims, np.full(tuple(reversed(dims)), 1.2))
+    g, N = 3e-5, 25
+    cpuref.legacy_walls_run(st, 'D2E4' if len(dims) == 2 else 'D3E4', 0.0, tau,
+                            np.ones(st.V, dtype=np.uint8), g, nsteps=N)
+    f = st.pop.reshape(st.Q, st.V)
+    J = [sum(v[d] * f[q].sum() for q, v in enumerate(XI[xi])) for d in range(len(dims))]
+    assert abs(J[1] - N * st.V * g) < 1e-12 * N * st.V * g
+    assert abs(J[0]) < 1e-15
+
+
+@pytest.mark.parametrize('xi,es,dims,g', [('D2Q9', 'D2E4', (23, 17), 0.0),
+                                          ('D2Q9', 'D2E4', (23, 17), 2e-4),
+                                          ('D3Q19', 'D3E4', (9, 8, 7), 1e-4)])
+def test_legacy_walls_fluid_sites_match_push_bounce_back(xi, es, dims, g):
+    """Pull streaming + K_HalfWayBounceBack and the push stream with half-way
+    bounce-back give the same fluid-site populations; with gravity, against
+    the numpy gather with the body force."""
+    from oracle import cpuref, np_ref
+    walls, psi_w = _channel(dims, 7)
+    n0 = droplet_field(dims, seed=52)
+    st = _state(xi, dims, n0)
+    st.psi[:] = psi_w.ravel()
+    shape = (st.Q,) + walls.shape
+    f = st.pop.reshape(shape).copy()
+    old = st.pop_swap.reshape(shape).copy()
+    for k in range(5):
+        f_new, n, psi = np_ref.step_walls(f, xi, es, G, 1.0, walls, psi_w, old, gy=g)
+        old, f = f, f_new
+    cpuref.legacy_walls_run(st, es, G, 1.0, walls.ravel(), g, nsteps=5)
+    fl = walls == 1
+    assert rel_err(st.pop.reshape(shape)[:, fl].ravel(), f[:, fl].ravel()) < 1e-13
+    # the legacy loop zeroes the solid sites' populations after bouncing them
+    assert not st.pop.reshape(shape)[:, ~fl].any() or np.abs(
+        st.pop.reshape(shape)[0, ~fl]).max() == 0.0
+    if g == 0.0:
+        mt = _state(xi, dims, n0)
+        mt.psi[:] = psi_w.ravel()
+        cpuref.meta_run(mt, es, G, 1.0, nsteps=5, walls=walls.ravel())
+        assert rel_err(st.pop.reshape(shape)[:, fl].ravel(),
+                       mt.pop.reshape(shape)[:, fl].ravel()) < 1e-13
