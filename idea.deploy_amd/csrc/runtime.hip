@@ -330,7 +330,8 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
   CHECK_ARG(d->estencil == IDMX_E4 || d->estencil == IDMX_E6 || d->estencil == IDMX_E8,
             "idmx_sc_create: unknown E-stencil");
   CHECK_ARG(d->psi_kind >= 0 && d->psi_kind <= 2, "idmx_sc_create: unknown psi kind");
-  CHECK_ARG(d->collision == IDMX_BGK || d->collision == IDMX_MRT,
+  CHECK_ARG(d->collision == IDMX_BGK || d->collision == IDMX_MRT ||
+                d->collision == IDMX_BGK_SCF,
             "idmx_sc_create: unknown collision");
   CHECK_ARG(d->precision == IDMX_PREC_F64 || d->precision == IDMX_PREC_F32S,
             "idmx_sc_create: unknown precision");
@@ -355,7 +356,9 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
   s->ctx = c;
   s->desc = *d;
   s->desc.mrt_A = s->desc.mrt_beta = s->desc.mrt_gamma = s->desc.mrt_diag = nullptr;
-  s->coll = d->collision == IDMX_BGK ? 0 : (d->mrt_diag ? 2 : 1);
+  s->coll = d->collision == IDMX_BGK       ? 0
+            : d->collision == IDMX_BGK_SCF ? 3
+                                           : (d->mrt_diag ? 2 : 1);
   if (D == 2) s->desc.dims[2] = 1;
   if (d->precision == IDMX_PREC_F64) s->desc.rho0 = 0.0;
   s->D = D;

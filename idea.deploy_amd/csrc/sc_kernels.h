@@ -135,6 +135,38 @@ __device__ inline void collide(const T (&f)[S::Q], T dn, T gx, T gy, T gz,
     Fy += by;
     if (S::D == 3) Fz += bz;
   }
+  if constexpr (COLL == 3) {
+    // BGK with the Shan-Chen forcing (K_Collision_ShanChenMultiPhase,
+    // idpy/LBM/MultiPhaseKernels.py:132-216): the equilibrium velocity is
+    // u + F/(n omega), no Guo term, and the velocity written is the mean of
+    // the pre- and post-collision velocities.
+    const T om = P.omega;
+    const T u0x = jx / n, u0y = jy / n, u0z = S::D == 3 ? jz / n : T(0);
+    const T lux = u0x + Fx / n / om, luy = u0y + Fy / n / om;
+    const T luz = S::D == 3 ? u0z + Fz / n / om : T(0);
+    const T uu = lux * lux + luy * luy + luz * luz;
+    T px = 0, py = 0, pz = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const T xu = S::cx[q] * lux + S::cy[q] * luy + S::cz[q] * luz;
+      T e = T(1);
+      e += xu * T(kCM2) + T(0.5) * xu * xu * T(kCM4);
+      e += T(-0.5) * uu * T(kCM2);
+      T feq = e * n * T(S::w[q]);
+      if (SHIFT) feq -= T(S::w[q]) * P.rho0;
+      fo[q] = f[q] * (T(1) - om) + feq * om;
+      if (S::cx[q] == 1) px += fo[q];
+      if (S::cx[q] == -1) px -= fo[q];
+      if (S::cy[q] == 1) py += fo[q];
+      if (S::cy[q] == -1) py -= fo[q];
+      if (S::cz[q] == 1) pz += fo[q];
+      if (S::cz[q] == -1) pz -= fo[q];
+    }
+    u[0] = T(0.5) * (u0x + px / n);
+    u[1] = T(0.5) * (u0y + py / n);
+    u[2] = S::D == 3 ? T(0.5) * (u0z + pz / n) : T(0);
+    return;
+  }
   const T inv_n = T(1) / n;
   const T ux = (jx + T(0.5) * Fx) * inv_n;
   const T uy = (jy + T(0.5) * Fy) * inv_n;

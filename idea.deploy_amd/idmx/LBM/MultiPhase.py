@@ -94,6 +94,30 @@ class ShanChenMultiPhase(_ShanChenCommon):
         finally:
             self.engine.set_walls(None)
 
+    def MainLoopSimpleSC(self, time_steps, convergence_functions=[], profiling=False):
+        """MainLoopSimpleSC (MultiPhase.py:257-349): BGK with the Shan-Chen
+        forcing (K_Collision_ShanChenMultiPhase, MultiPhaseKernels.py:132-216:
+        equilibrium velocity u + F/(n omega), no Guo term), pull streaming.
+        Writes n, psi and u = (u + u_post) / 2 as the legacy kernels do."""
+        m = self.sims_idpy_memory
+        return self._run_loop(time_steps, convergence_functions, profiling,
+                              self._scf_engine(), m['u'])
+
+    def MainLoopSimpleWallsSC(self, time_steps, convergence_functions=[], profiling=False):
+        """MainLoopSimpleWallsSC (MultiPhase.py:467-581): MainLoopSimpleSC on
+        the fluid sites of the walls set by GetWalls, with half-way
+        bounce-back (K_Collision_ShanChenMultiPhaseWalls,
+        MultiPhaseKernels.py:218-304; K_HalfWayBounceBack)."""
+        m = self.sims_idpy_memory
+        if 'walls' not in m:
+            raise Exception("Missing 'walls': call GetWalls first")
+        eng = self._scf_engine()
+        eng.set_walls(m['walls'])
+        try:
+            return self._run_loop(time_steps, convergence_functions, profiling, eng, m['u'])
+        finally:
+            eng.set_walls(None)
+
     def MainLoopSimpleWalls(self, time_steps, convergence_functions=[], profiling=False):
         """MainLoopSimpleWalls (MultiPhase.py:351-465): the legacy-kernel walls
         loop (K_ComputeMomentsWalls, K_ComputePsiWalls,

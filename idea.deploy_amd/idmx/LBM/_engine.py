@@ -20,7 +20,7 @@ RUN_TWO_PASS = 2
 
 class SCEngine:
     def __init__(self, tenet, xi_name, e_name, dims, psi_kind, G, tau,
-                 precision='fp64', rho0=0.0, mrt=None):
+                 precision='fp64', rho0=0.0, mrt=None, forcing='guo'):
         self.tenet = tenet
         self.xi_name, self.e_name = xi_name, e_name
         self.dims = tuple(int(d) for d in dims)
@@ -39,7 +39,10 @@ class SCEngine:
         d.psi_kind = PSI_KINDS[psi_kind]
         d.SC_G = float(G)
         d.tau = float(tau)
-        d.collision = 0 if mrt is None else 1
+        if forcing not in ('guo', 'sc') or (forcing == 'sc' and mrt is not None):
+            raise ValueError("forcing is 'guo', or 'sc' (Shan-Chen forcing, BGK only)")
+        self.forcing = forcing
+        d.collision = (0 if forcing == 'guo' else 2) if mrt is None else 1
         d.precision = PRECISIONS[precision]
         d.rho0 = self.rho0
         keep = []

@@ -50,7 +50,13 @@ enum {
   IDMX_PSI_EXPINV = 1, /* exp(-1/n)   */
   IDMX_PSI_LINEAR = 2  /* n           */
 };
-enum { IDMX_BGK = 0, IDMX_MRT = 1 };
+/* Collisions: BGK and MRT with the Guo forcing; IDMX_BGK_SCF is BGK with
+ * the Shan-Chen forcing (equilibrium velocity u + F/(n omega), no Guo term,
+ * u written as the mean of the pre- and post-collision velocities;
+ * K_Collision_ShanChenMultiPhase[Walls], idpy/LBM/MultiPhaseKernels.py:
+ * 132-304, used by MainLoopSimpleSC / MainLoopSimpleWallsSC), which runs
+ * on the two-pass step. */
+enum { IDMX_BGK = 0, IDMX_MRT = 1, IDMX_BGK_SCF = 2 };
 enum { IDMX_PREC_F64 = 0, IDMX_PREC_F32S = 1 };
 
 /* Flags of idmx_sc_run. */
@@ -74,7 +80,7 @@ typedef struct {
   int psi_kind;       /* IDMX_PSI_*                                    */
   double SC_G;        /* coupling G                                    */
   double tau;         /* BGK relaxation time; OMEGA = 1/tau            */
-  int collision;      /* IDMX_BGK | IDMX_MRT                           */
+  int collision;      /* IDMX_BGK | IDMX_MRT | IDMX_BGK_SCF            */
   int precision;      /* IDMX_PREC_F64 | IDMX_PREC_F32S                */
   double rho0;        /* population shift for IDMX_PREC_F32S           */
   /* MRT only (collision == IDMX_MRT), row-major float64 tables built on

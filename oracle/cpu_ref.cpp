@@ -32,7 +32,9 @@
 //   * MainLoopSimpleWalls / MainLoopSimpleWallsGravity (idpy/LBM/MultiPhase.py:
 //     351-465, 888-1010): the legacy kernels on the flagged sites, pull
 //     streaming, then K_HalfWayBounceBack (idpy/LBM/LBMKernels.py:151-184)
-//     and a body force F[1] += g;
+//     and a body force F[1] += g; MainLoopSimpleSC / MainLoopSimpleWallsSC
+//     (MultiPhase.py:257-349, 467-581) with the Shan-Chen forcing collision
+//     (K_Collision_ShanChenMultiPhase[Walls], MultiPhaseKernels.py:132-304);
 //   * ComputeMomentsMeta (idpy/LBM/MultiPhase.py:1565-1662,
 //     K_ComputeVelocityAfterForceSCMPMeta MultiPhaseKernelsMeta.py:1077-1219)
 //   * InitPopulations (K_InitPopulations idpy/LBM/LBM.py:1700-1743)
@@ -234,14 +236,19 @@ void legacy_step(const Geo& geo, const EStencil& es, double G, double omega,
 template <class S>
 int opposite(int q);
 
+// scf: the Shan-Chen forcing of MainLoopSimpleSC / MainLoopSimpleWallsSC
+// (K_Collision_ShanChenMultiPhase[Walls], MultiPhaseKernels.py:132-304) in
+// place of the Guo collision. walls == nullptr: no walls (every site fluid,
+// no bounce-back pass), as in MainLoopSimpleSC (MultiPhase.py:257-349).
 template <class S>
 void legacy_walls_step(const Geo& geo, const EStencil& es, double G, double omega,
-                       int psi_kind, const uint8_t* walls, double gy, double*& pop,
-                       double*& pop_swap, double* n, double* u, double* psi) {
+                       int psi_kind, const uint8_t* walls, double gy, bool scf,
+                       double*& pop, double*& pop_swap, double* n, double* u, double* psi) {
   constexpr int D = S::D, Q = S::Q;
   const int64_t V = geo.V;
+  auto fluid = [&](int64_t g) { return !walls || walls[g] == 1; };
   for (int64_t g = 0; g < V; g++) {
-    if (walls[g] != 1) continue;
+    if (!fluid(g)) continue;
     double lu[3] = {0., 0., 0.};
     double ln = 0.;
     for (int q = 0; q < Q; q++) {
@@ -253,11 +260,32 @@ void legacy_walls_step(const Geo& geo, const EStencil& es, double G, double omeg
     for (int d = 0; d < D; d++) u[g + d * V] = lu[d] / ln;
   }
   for (int64_t g = 0; g < V; g++)
-    if (walls[g]) psi[g] = psi_of(psi_kind, n[g]);
+    if (!walls || walls[g]) psi[g] = psi_of(psi_kind, n[g]);
   for (int64_t g = 0; g < V; g++) {
-    if (!walls[g]) continue;
+    if (scf ? !fluid(g) : (walls && !walls[g])) continue;
     double F[3];
     sc_force<D>(geo, es, psi, g, G, F);
+    if (scf) {
+      // K_Collision_ShanChenMultiPhase: u_eq = u + F / n / omega, BGK
+      // without a forcing term, u <- (u + sum xi f* / n) / 2
+      const double ln = n[g];
+      double lu[3], lu_post[3] = {0., 0., 0.};
+      for (int d = 0; d < D; d++) lu[d] = u[g + V * d] + F[d] / ln / omega;
+      double u_dot_u = 0.;
+      for (int d = 0; d < D; d++) u_dot_u += lu[d] * lu[d];
+      for (int q = 0; q < Q; q++) {
+        double u_dot_xi = 0.;
+        for (int d = 0; d < D; d++) u_dot_xi += lu[d] * S::xi[q][d];
+        double leq = 1.;
+        leq += +u_dot_xi * kCM2 + 0.5 * u_dot_xi * u_dot_xi * kCM4;
+        leq += -0.5 * u_dot_u * kCM2;
+        leq = leq * ln * S::w[q];
+        pop[g + q * V] = pop[g + q * V] * (1. - omega) + leq * omega;
+        for (int d = 0; d < D; d++) lu_post[d] += pop[g + q * V] * S::xi[q][d];
+      }
+      for (int d = 0; d < D; d++) u[g + V * d] = 0.5 * (u[g + V * d] + lu_post[d] / ln);
+      continue;
+    }
     F[1] += gy;
     const double ln = n[g];
     double lu[3];
@@ -286,7 +314,7 @@ void legacy_walls_step(const Geo& geo, const EStencil& es, double G, double omeg
       pop_swap[g + q * V] = pop[geo.nb(p, e) + q * V];
     }
   }
-  for (int64_t g = 0; g < V; g++) {
+  for (int64_t g = 0; walls && g < V; g++) {
     if (walls[g] != 0) continue;
     int p[3];
     geo.pos(g, p);
@@ -490,21 +518,24 @@ int cpuref_legacy_run(int xi_id, const int* L, int QE, const int* E,
 }
 
 // Runs nsteps of MainLoopSimpleWalls (gy == 0) or MainLoopSimpleWallsGravity
-// (SoA); walls is V flags, 1 fluid / 0 solid. pop holds the state after the
-// call; n, u, psi hold the values of the last step.
+// (SoA), or with scf != 0 MainLoopSimpleWallsSC (MainLoopSimpleSC when walls
+// is NULL); walls is V flags, 1 fluid / 0 solid. pop holds the state after
+// the call; n, u, psi hold the values of the last step.
 int cpuref_legacy_walls_run(int xi_id, const int* L, int QE, const int* E,
                             const double* EW, double G, double omega, int psi_kind,
-                            const uint8_t* walls, double gy, double* pop, double* pop_swap,
-                            double* n, double* u, double* psi, int nsteps) {
+                            const uint8_t* walls, double gy, int scf, double* pop,
+                            double* pop_swap, double* n, double* u, double* psi, int nsteps) {
   Geo geo;
   if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
   EStencil es{QE, E, EW};
   double *a = pop, *b = pop_swap;
   for (int s = 0; s < nsteps; s++) {
     if (xi_id == 0)
-      legacy_walls_step<SD2Q9>(geo, es, G, omega, psi_kind, walls, gy, a, b, n, u, psi);
+      legacy_walls_step<SD2Q9>(geo, es, G, omega, psi_kind, walls, gy, scf != 0, a, b, n, u,
+                               psi);
     else
-      legacy_walls_step<SD3Q19>(geo, es, G, omega, psi_kind, walls, gy, a, b, n, u, psi);
+      legacy_walls_step<SD3Q19>(geo, es, G, omega, psi_kind, walls, gy, scf != 0, a, b, n, u,
+                                psi);
   }
   const int Q = xi_id == 0 ? 9 : 19;
   if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);

@@ -35,7 +35,7 @@ def lib():
                                           C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p]
         L.cpuref_legacy_walls_run.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
                                               C.c_double, C.c_int, C.c_void_p, C.c_double,
-                                              _dp, _dp, _dp, _dp, _dp, C.c_int]
+                                              C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int]
         L.cpuref_init_pop.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp]
         L.cpuref_init_radial.argtypes = [C.c_int, _ip, C.c_double, C.c_double,
                                          C.c_double, C.c_double, _dp, _dp]
@@ -104,16 +104,18 @@ def legacy_run(st, estencil, G, tau, psi='exp', nsteps=1):
     assert rc == 0
 
 
-def legacy_walls_run(st, estencil, G, tau, walls, g=0.0, psi='exp', nsteps=1):
+def legacy_walls_run(st, estencil, G, tau, walls, g=0.0, psi='exp', nsteps=1, forcing='guo'):
     """MainLoopSimpleWalls (g = 0) / MainLoopSimpleWallsGravity
-    (idpy/LBM/MultiPhase.py:351-465, 888-1010), nsteps; SoA; st.psi must hold
-    the solid sites' psi (GetWalls)."""
-    assert not st.aos
+    (idpy/LBM/MultiPhase.py:351-465, 888-1010), or with forcing='sc'
+    MainLoopSimpleWallsSC (:467-581; walls None: MainLoopSimpleSC,
+    :257-349), nsteps; SoA; st.psi must hold the solid sites' psi
+    (GetWalls)."""
+    assert not st.aos and forcing in ('guo', 'sc')
     QE, Ei, Ew = _estencil(estencil)
     wkeep, wptr = _walls_ptr(walls, st.V)
     rc = lib().cpuref_legacy_walls_run(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, 1.0 / tau,
-                                       PSI_KIND[psi], wptr, float(g), st.pop, st.pop_swap,
-                                       st.n, st.u, st.psi, nsteps)
+                                       PSI_KIND[psi], wptr, float(g), int(forcing == 'sc'),
+                                       st.pop, st.pop_swap, st.n, st.u, st.psi, nsteps)
     assert rc == 0
 
 

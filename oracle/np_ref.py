@@ -109,6 +109,29 @@ def step_walls(f, xi, estencil, G, tau, walls, psi_solid, f_old, psi_kind='exp',
     return fn, np.where(fluid, n, 0.0), psi
 
 
+def step_sc(f, xi, estencil, G, tau, psi_kind='exp'):
+    """One periodic step with the Shan-Chen forcing (MainLoopSimpleSC,
+    idpy/LBM/MultiPhase.py:257-349; K_Collision_ShanChenMultiPhase,
+    MultiPhaseKernels.py:132-216): BGK towards f_eq(n, u + F/(n omega)),
+    pull streaming. Returns (f_new, n, psi, u_written) with u_written the
+    mean of the pre- and post-collision velocities."""
+    omega = 1.0 / tau
+    xis = XI[xi]
+    D = len(xis[0])
+    n = f.sum(axis=0)
+    psi = psi_fn(n, psi_kind)
+    F = sc_force(psi, estencil, G)
+    u0 = [sum(xis[q][d] * f[q] for q in range(len(xis))) / n for d in range(D)]
+    ueq = [u0[d] + F[d] / (n * omega) for d in range(D)]
+    feq, _ = feq_guo(xi, n, ueq, [np.zeros_like(n)] * D)
+    fs = (1 - omega) * f + omega * feq
+    upost = [sum(xis[q][d] * fs[q] for q in range(len(xis))) / n for d in range(D)]
+    fn = np.empty_like(f)
+    for q, v in enumerate(xis):
+        fn[q] = _shift(fs[q], tuple(-c for c in v))
+    return fn, n, psi, [0.5 * (u0[d] + upost[d]) for d in range(D)]
+
+
 def init_pop(xi, n, u=None):
     D = len(XI[xi][0])
     if u is None:

@@ -137,3 +137,46 @@ def test_simple_walls_gravity_loop_matches_oracle(tenet, xi, es, dims, g, nsteps
     print('rel err pop', e_pop, 'n', e_n, 'psi', e_psi, 'u', e_u, flush=True)
     assert e_pop <= 1e-12 and e_n <= 1e-12 and e_psi <= 1e-12
     assert e_u <= 1e-11
+
+
+@pytest.mark.parametrize('xi,es,dims,walled,nsteps', [('D2Q9', 'D2E4', (64, 37), False, 200),
+                                                      ('D2Q9', 'D2E4', (64, 37), True, 200),
+                                                      ('D3Q19', 'D3E4', (21, 18, 13), True, 40)])
+def test_sc_forcing_loops_match_oracle(tenet, xi, es, dims, walled, nsteps):
+    """MainLoopSimpleSC / MainLoopSimpleWallsSC (BGK with the Shan-Chen
+    forcing) against the oracle: fluid-site populations, n, psi and the
+    velocity the loops write, (u + u_post) / 2."""
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    from oracle import cpuref
+    walls, psi_w = _geometry(dims, 9)
+    n0 = droplet_field(dims, seed=48)
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=dims, xi_stencil=XIStencils[xi],
+                             f_stencil=FStencils[es], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=G, tau=1.0)
+    sim.SetDensityField(n0)
+    st = cpuref.State(xi, dims)
+    cpuref.init_pop(st, n0.ravel(), np.zeros(st.V * st.D))
+    if walled:
+        sim.GetWalls(walls, psi_w)
+        sim.MainLoopSimpleWallsSC([0, nsteps // 2, nsteps])
+        st.psi[:] = psi_w.ravel()
+        cpuref.legacy_walls_run(st, es, G, 1.0, walls.ravel(), forcing='sc', nsteps=nsteps)
+        fl = walls.ravel() == 1
+    else:
+        sim.MainLoopSimpleSC([0, nsteps])
+        cpuref.legacy_walls_run(st, es, G, 1.0, None, forcing='sc', nsteps=nsteps)
+        fl = np.ones(st.V, dtype=bool)
+    Q, V, D = st.Q, st.V, st.D
+    m = sim.sims_idpy_memory
+    pop = m['pop'].D2H().reshape(Q, V)
+    e_pop = rel_err(pop[:, fl].ravel(), st.pop.reshape(Q, V)[:, fl].ravel())
+    e_n = rel_err(m['n'].D2H()[fl], st.n[fl])
+    e_psi = rel_err(m['psi'].D2H()[fl], st.psi[fl])
+    e_u = rel_err(m['u'].D2H().reshape(D, V)[:, fl].ravel(),
+                  st.u.reshape(D, V)[:, fl].ravel())
+    sim.End()
+    print('rel err pop', e_pop, 'n', e_n, 'psi', e_psi, 'u', e_u, flush=True)
+    assert e_pop <= 1e-12 and e_n <= 1e-12 and e_psi <= 1e-12
+    assert e_u <= 1e-11
