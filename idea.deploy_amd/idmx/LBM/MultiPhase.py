@@ -87,12 +87,9 @@ class ShanChenMultiPhase(_ShanChenCommon):
         m = self.sims_idpy_memory
         if 'walls' not in m:
             raise Exception("Missing 'walls': call GetWalls first")
-        self.engine.set_walls(m['walls'])
-        try:
+        with self.engine.configured(walls=m['walls']):
             return self._run_loop(time_steps, convergence_functions, profiling, self.engine,
                                   self._u_step)
-        finally:
-            self.engine.set_walls(None)
 
     def MainLoopSimpleSC(self, time_steps, convergence_functions=[], profiling=False):
         """MainLoopSimpleSC (MultiPhase.py:257-349): BGK with the Shan-Chen
@@ -112,11 +109,8 @@ class ShanChenMultiPhase(_ShanChenCommon):
         if 'walls' not in m:
             raise Exception("Missing 'walls': call GetWalls first")
         eng = self._scf_engine()
-        eng.set_walls(m['walls'])
-        try:
+        with eng.configured(walls=m['walls']):
             return self._run_loop(time_steps, convergence_functions, profiling, eng, m['u'])
-        finally:
-            eng.set_walls(None)
 
     def MainLoopSimpleWalls(self, time_steps, convergence_functions=[], profiling=False):
         """MainLoopSimpleWalls (MultiPhase.py:351-465): the legacy-kernel walls
@@ -137,14 +131,9 @@ class ShanChenMultiPhase(_ShanChenCommon):
         m = self.sims_idpy_memory
         if 'walls' not in m:
             raise Exception("Missing 'walls': call GetWalls first")
-        self.engine.set_walls(m['walls'])
-        self.engine.set_body_force(0.0, float(g), 0.0)
-        try:
+        with self.engine.configured(walls=m['walls'], body=(0.0, float(g), 0.0)):
             return self._run_loop(time_steps, convergence_functions, profiling, self.engine,
                                   m['u'])
-        finally:
-            self.engine.set_walls(None)
-            self.engine.set_body_force(0.0, 0.0, 0.0)
 
     def GetWalls(self, walls, psi_w):
         """psi_w (V values, the psi of the solid sites; the fluid sites' are
@@ -173,14 +162,11 @@ class ShanChenMultiPhase(_ShanChenCommon):
         if not self.init_status['pop']:
             raise Exception('Populations are not initialized')
         m = self.sims_idpy_memory
-        walls = m.get('walls')
-        if walls is not None:
-            self.engine.set_walls(walls)
-        try:
+        # as on the reference: the walls whenever they exist, never a body
+        # force (the Meta moments kernels have none); a wall loop calling this
+        # from a convergence callback gets its own settings back afterwards
+        with self.engine.configured(walls=m.get('walls'), body=(0.0, 0.0, 0.0)):
             self.engine.moments(m['pop'], m['n'], m['psi'], m['u'])
-        finally:
-            if walls is not None:
-                self.engine.set_walls(None)
         self.init_status['n'] = True
         self.init_status['u'] = True
 

@@ -3,6 +3,7 @@
 Only parameter marshalling and device pointers live here; every arithmetic
 operation on lattice data runs in the HIP kernels.
 """
+import contextlib
 import ctypes as C
 
 import numpy as np
@@ -16,6 +17,9 @@ PRECISIONS = {'fp64': 0, 'fp32s': 1}
 
 RUN_WRITE_FIELDS = 1
 RUN_TWO_PASS = 2
+
+
+_KEEP = object()  # "leave this setting as it is" for SCEngine.configured
 
 
 class SCEngine:
@@ -124,11 +128,31 @@ class SCEngine:
         """walls: device array of V uint8 flags (1 fluid, 0 solid), or None.
         While set, run / moments step as MainLoopWalls."""
         _lib.call('idmx_sc_set_walls', self.handle, walls.ptr if walls is not None else None)
+        self._walls = walls
 
     def set_body_force(self, fx=0.0, fy=0.0, fz=0.0):
         """Body force added to the Shan-Chen force (gravity of
         MainLoopSimpleWallsGravity); zero restores the plain step."""
         _lib.call('idmx_sc_set_body_force', self.handle, float(fx), float(fy), float(fz))
+        self._body = (float(fx), float(fy), float(fz))
+
+    @contextlib.contextmanager
+    def configured(self, walls=_KEEP, body=_KEEP):
+        """Walls and / or body force for the duration of a with-block; the
+        previous settings are restored afterwards (so a convergence callback
+        that computes moments inside a wall loop leaves the loop's walls in
+        place)."""
+        old_w = getattr(self, '_walls', None)
+        old_b = getattr(self, '_body', (0.0, 0.0, 0.0))
+        try:
+            if walls is not _KEEP:
+                self.set_walls(walls)
+            if body is not _KEEP:
+                self.set_body_force(*body)
+            yield self
+        finally:
+            self.set_walls(old_w)
+            self.set_body_force(*old_b)
 
     def launch_ok(self, tx, ty, H, hist=1):
         """Whether set_launch accepts this shape (variant 9 needs its LDS

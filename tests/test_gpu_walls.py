@@ -180,3 +180,48 @@ def test_sc_forcing_loops_match_oracle(tenet, xi, es, dims, walled, nsteps):
     print('rel err pop', e_pop, 'n', e_n, 'psi', e_psi, 'u', e_u, flush=True)
     assert e_pop <= 1e-12 and e_n <= 1e-12 and e_psi <= 1e-12
     assert e_u <= 1e-11
+
+
+def test_wall_loops_keep_walls_across_moment_callbacks(tenet):
+    """A convergence callback that calls ComputeMomentsMeta between
+    checkpoints (as CheckUConvergenceSCMP does) must not change the step of
+    MainLoopWalls / MainLoopSimpleWallsGravity."""
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    from oracle import cpuref
+    dims = (48, 31)
+    walls, psi_w = _geometry(dims, 10)
+    n0 = droplet_field(dims, seed=49)
+    n = sp.Symbol('n')
+    calls = []
+
+    def cb(lbm):
+        lbm.ComputeMomentsMeta()
+        calls.append(1)
+        return False
+
+    for loop, g in (('MainLoopWalls', 0.0), ('MainLoopSimpleWallsGravity', 3e-5)):
+        sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=dims, xi_stencil=XIStencils['D2Q9'],
+                                 f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln_0)',
+                                 psi_sym=1 - sp.exp(-n), SC_G=G, tau=1.0)
+        sim.SetDensityField(n0)
+        sim.GetWalls(walls, psi_w)
+        steps = list(range(0, 61, 20))
+        if g:
+            sim.MainLoopSimpleWallsGravity(steps, convergence_functions=[cb], g=g)
+        else:
+            sim.MainLoopWalls(steps, convergence_functions=[cb])
+        st = cpuref.State('D2Q9', dims)
+        cpuref.init_pop(st, n0.ravel(), np.zeros(st.V * st.D))
+        st.psi[:] = psi_w.ravel()
+        if g:
+            cpuref.legacy_walls_run(st, 'D2E4', G, 1.0, walls.ravel(), g, nsteps=60)
+        else:
+            cpuref.meta_run(st, 'D2E4', G, 1.0, nsteps=60, walls=walls.ravel())
+        fl = walls.ravel() == 1
+        pop = sim.sims_idpy_memory['pop'].D2H().reshape(st.Q, st.V)
+        err = rel_err(pop[:, fl].ravel(), st.pop.reshape(st.Q, st.V)[:, fl].ravel())
+        sim.End()
+        print(loop, 'rel err', err, flush=True)
+        assert err <= 1e-12
+    assert len(calls) == 6
