@@ -12,6 +12,10 @@
  *     (idpy/Utils/HostModule.py:273-285, idpy/Utils/CTypesTypes.py:63-66).
  *   - Work is enqueued on the context's HIP stream. idmx_d2h blocks until the
  *     copy is done; idmx_sync waits for all enqueued work.
+ *   - A context, and the simulations and batches created on it, may be used
+ *     by one host thread at a time: they share the context's stream and its
+ *     reduction scratch buffers. Use one context per thread (or serialise the
+ *     calls) for concurrent work; separate contexts are independent.
  *   - Lattice index g = x + y*L0 (+ z*L0*L1), x fastest
  *     (idpy/IdpyCode/IdpyUnroll.py:325-386). Populations are SoA,
  *     pop[g + q*V] (the reference's 'gpu' ordering, idpy/LBM/LBM.py:529-537).
