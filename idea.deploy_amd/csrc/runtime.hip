@@ -331,8 +331,11 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
             "idmx_sc_create: unknown E-stencil");
   CHECK_ARG(d->psi_kind >= 0 && d->psi_kind <= 2, "idmx_sc_create: unknown psi kind");
   CHECK_ARG(d->collision == IDMX_BGK || d->collision == IDMX_MRT ||
-                d->collision == IDMX_BGK_SCF,
+                d->collision == IDMX_BGK_SCF || d->collision == IDMX_BGK_ISO,
             "idmx_sc_create: unknown collision");
+  CHECK_ARG(d->collision != IDMX_BGK_ISO ||
+                (d->xi == IDMX_D2Q9 && d->precision == IDMX_PREC_F64),
+            "idmx_sc_create: the isotropy filter needs D2Q9 and FP64");
   CHECK_ARG(d->precision == IDMX_PREC_F64 || d->precision == IDMX_PREC_F32S,
             "idmx_sc_create: unknown precision");
   CHECK_ARG(d->tau > 0, "idmx_sc_create: tau must be positive");
@@ -358,6 +361,7 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
   s->desc.mrt_A = s->desc.mrt_beta = s->desc.mrt_gamma = s->desc.mrt_diag = nullptr;
   s->coll = d->collision == IDMX_BGK       ? 0
             : d->collision == IDMX_BGK_SCF ? 3
+            : d->collision == IDMX_BGK_ISO ? 4
                                            : (d->mrt_diag ? 2 : 1);
   if (D == 2) s->desc.dims[2] = 1;
   if (d->precision == IDMX_PREC_F64) s->desc.rho0 = 0.0;

@@ -160,7 +160,7 @@ struct Impl {
     // with walls the step is the two-pass one (density and psi of the fluid
     // sites into n / psi, where psi also holds the solid sites' values)
     const bool body = s->bforce[0] != 0.0 || s->bforce[1] != 0.0 || s->bforce[2] != 0.0;
-    const bool twop = (flags & IDMX_RUN_TWO_PASS) != 0 || s->walls || body || COLL == 3;
+    const bool twop = (flags & IDMX_RUN_TWO_PASS) != 0 || s->walls || body || COLL >= 3;
     if (twop && (!n || !psi)) {
       set_error(s->walls || body
                     ? "idmx_sc_run: walls or a body force need the n and psi buffers"
@@ -177,7 +177,7 @@ struct Impl {
     // step in it, convert back (nsteps + 2 buffer swaps: same parity as
     // nsteps, so the caller's pointer bookkeeping is unchanged)
     const bool pull = !twop && pull_form(s->hist) && nsteps > 0;
-    if constexpr (COLL != 3) if (pull) {
+    if constexpr (COLL < 3) if (pull) {
       hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
                          (const T*)*pop, (T*)*swap, s->geo, 1);
       IDMX_HIP(hipGetLastError());
@@ -188,8 +188,9 @@ struct Impl {
       const T* a = (const T*)*pop;
       T* b = (T*)*swap;
       hipError_t e;
-      if constexpr (COLL == 3) {
-        // the Shan-Chen forcing runs on the two-pass kernels only
+      if constexpr (COLL >= 3) {
+        // the Shan-Chen forcing and the isotropy filter run on the two-pass
+        // kernels only
         e = two_pass(s, a, b, n, psi, u, wf && last);
       } else {
         if (twop) e = two_pass(s, a, b, n, psi, u, wf && last);
@@ -199,7 +200,7 @@ struct Impl {
       if (e != hipSuccess) return hip_fail(e, "idmx_sc_run launch");
       std::swap(*pop, *swap);
     }
-    if constexpr (COLL != 3) if (pull) {
+    if constexpr (COLL < 3) if (pull) {
       hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
                          (const T*)*pop, (T*)*swap, s->geo, -1);
       IDMX_HIP(hipGetLastError());
@@ -337,9 +338,14 @@ struct Impl {
 // two drift apart by 2e-11 over the same run. The FP32-shifted step keeps
 // contraction (its accuracy is set by FP32 storage).
 // coll: 0 BGK, 1 MRT with dense tables, 2 MRT with a diagonal S, 3 BGK with
-// the Shan-Chen forcing (two-pass step only).
+// the Shan-Chen forcing, 4 BGK followed by the isotropy filter (D2Q9 and
+// FP64 only; 3 and 4 on the two-pass step only).
 template <class S, class ES>
 const SimOps* ops_for_f64(int coll) {
+  if (coll == 4) {
+    if constexpr (S::Q == 9) return Impl<S, ES, double, false, 4>::ops();
+    else return nullptr;
+  }
   if (coll == 0) return Impl<S, ES, double, false, 0>::ops();
   if (coll == 1) return Impl<S, ES, double, false, 1>::ops();
   if (coll == 3) return Impl<S, ES, double, false, 3>::ops();
@@ -347,6 +353,7 @@ const SimOps* ops_for_f64(int coll) {
 }
 template <class S, class ES>
 const SimOps* ops_for_f32(int coll) {
+  if (coll == 4) return nullptr;
   if (coll == 0) return Impl<S, ES, float, true, 0>::ops();
   if (coll == 1) return Impl<S, ES, float, true, 1>::ops();
   if (coll == 3) return Impl<S, ES, float, true, 3>::ops();

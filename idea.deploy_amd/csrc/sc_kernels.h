@@ -106,6 +106,47 @@ __device__ constexpr bool mpv_row_nz(int k) {
 }
 
 // --------------------------------------------------------------------------
+// Isotropy filter of one D2Q9 site's post-collision populations, in place
+// (K_IsotropyFilter, idpy/LBM/LBMKernelsMeta.py:895-1080): Hermite moments
+// m_k = sum_q IsoM[k][q] f_q (rows 1, x, y, xx, xy, yy, xxy, xyy, xxyy)
+// normalised by m_0; the raw xy, xx - yy, xxy and xyy sums set from the
+// other moments (:1003-1015); the populations solved back as IsoAi b (IsoAi
+// the inverse of the system's matrix) and rescaled by m_0.
+template <class S, class T>
+__device__ inline void iso_filter(T (&f)[S::Q]) {
+  static_assert(S::Q == 9 && S::D == 2, "the isotropy filter is defined for D2Q9");
+  T m[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    T a = 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+      if (S::IsoM[k][q] != 0.0) a += T(S::IsoM[k][q]) * f[q];
+    m[k] = a;
+  }
+#pragma unroll
+  for (int k = 1; k < 9; k++) m[k] = m[k] / m[0];
+  const T third = T(1) / T(3);
+  const T b[9] = {T(1),
+                  m[1],
+                  m[2],
+                  m[3],
+                  m[1] * m[2],
+                  m[1] * m[1] - m[2] * m[2],
+                  T(2) * m[4] * m[1] + (m[3] + third) * m[2] - T(2) * m[2] * m[1] * m[1],
+                  T(2) * m[4] * m[2] + (m[5] + third) * m[1] - T(2) * m[1] * m[2] * m[2],
+                  m[8]};
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    T a = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      if (S::IsoAi[q][k] != 0.0) a += T(S::IsoAi[q][k]) * b[k];
+    f[q] = a * m[0];
+  }
+}
+
+// --------------------------------------------------------------------------
 // Collision of one site. f holds the (shifted when SHIFT) populations, dn is
 // sum_q f_q of those values, (gx, gy, gz) the isotropic gradient of psi and
 // lpsi = psi(x). Returns f* in fo and the Guo-shifted velocity in u. With BF
@@ -176,7 +217,7 @@ __device__ inline void collide(const T (&f)[S::Q], T dn, T gx, T gy, T gz,
   u[2] = uz;
   const T uu = ux * ux + uy * uy + uz * uz;
   const T Fu = Fx * ux + Fy * uy + Fz * uz;
-  if (COLL == 0) {
+  if (COLL == 0 || COLL == 4) {
     const T om = P.omega;
     const T c_f = T(1) - om;
     const T c_g = T(1) - T(0.5) * om;
@@ -192,6 +233,8 @@ __device__ inline void collide(const T (&f)[S::Q], T dn, T gx, T gy, T gz,
       const T fg = w * (T(kCM2) * xF + T(kCM4) * xF * xu - T(kCM2) * Fu);
       fo[q] = c_f * f[q] + om * feq + c_g * fg;
     }
+    // COLL 4: BGK with Guo forcing, then the isotropy filter (D2Q9, FP64)
+    if constexpr (COLL == 4) iso_filter<S, T>(fo);
   } else if (COLL == 2) {
     // Diagonal S in moment space with the bases as compile-time tables:
     // f* = f - Mw^-1 [s o (Mw f - Mw f_eq)] + Mp^-1 [(1 - s/2) o (Mp F)],

@@ -91,6 +91,19 @@ class ShanChenMultiPhase(_ShanChenCommon):
             return self._run_loop(time_steps, convergence_functions, profiling, self.engine,
                                   self._u_step)
 
+    def MainLoopIsoFilter(self, time_steps, convergence_functions=[], profiling=False):
+        """MainLoopIsoFilter (MultiPhase.py:1099-1223), D2Q9: the Meta step
+        with the isotropy filter (K_IsotropyFilter, LBMKernelsMeta.py:
+        895-1080) applied to every site's post-collision populations before
+        the push stream. FP64 only (the filter is written for nine FP64
+        populations). Runs the two-pass step; u is not written."""
+        if self.xi_name != 'D2Q9':
+            raise NotImplementedError('MainLoopIsoFilter is defined for D2Q9')
+        if self.params_dict['precision'] != 'fp64':
+            raise NotImplementedError('MainLoopIsoFilter runs in FP64')
+        return self._run_loop(time_steps, convergence_functions, profiling,
+                              self._iso_engine(), self._u_step)
+
     def MainLoopSimpleSC(self, time_steps, convergence_functions=[], profiling=False):
         """MainLoopSimpleSC (MultiPhase.py:257-349): BGK with the Shan-Chen
         forcing (K_Collision_ShanChenMultiPhase, MultiPhaseKernels.py:132-216:

@@ -323,19 +323,25 @@ class _ShanChenCommon(RootLB):
         self.sims_vars['grid'] = ((self.sims_vars['V'] + bs - 1) // bs, 1, 1)
         self.sims_vars['block'] = (bs, 1, 1)
 
-    def _make_engine(self, mrt=None, forcing='guo'):
+    def _make_engine(self, mrt=None, forcing='guo', iso_filter=False):
         rho0 = self.params_dict.get('rho0', 1.0)
         return SCEngine(self.tenet, self.xi_name, self.e_name,
                         tuple(int(s) for s in self.sims_vars['dim_sizes']), self.psi_kind,
                         self.params_dict['SC_G'], self.params_dict['tau'],
                         precision=self.params_dict['precision'], rho0=rho0, mrt=mrt,
-                        forcing=forcing)
+                        forcing=forcing, iso_filter=iso_filter)
 
     def _scf_engine(self):
         """Engine with the Shan-Chen forcing (BGK), created on first use."""
         if getattr(self, '_scf', None) is None:
             self._scf = self._make_engine(forcing='sc')
         return self._scf
+
+    def _iso_engine(self):
+        """Engine of BGK followed by the isotropy filter, created on first use."""
+        if getattr(self, '_iso', None) is None:
+            self._iso = self._make_engine(iso_filter=True)
+        return self._iso
 
     def _init_memory(self, pop_aos, u_aos):
         t = self.tenet
@@ -540,6 +546,8 @@ class _ShanChenCommon(RootLB):
             e.close()
         if getattr(self, '_scf', None) is not None:
             self._scf.close()
+        if getattr(self, '_iso', None) is not None:
+            self._iso.close()
         if getattr(self, 'engine', None) is not None:
             self.engine.close()
         self.tenet.FreeMemoryDict(memory_dict=self.sims_idpy_memory)

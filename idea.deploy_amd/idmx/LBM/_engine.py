@@ -24,7 +24,7 @@ _KEEP = object()  # "leave this setting as it is" for SCEngine.configured
 
 class SCEngine:
     def __init__(self, tenet, xi_name, e_name, dims, psi_kind, G, tau,
-                 precision='fp64', rho0=0.0, mrt=None, forcing='guo'):
+                 precision='fp64', rho0=0.0, mrt=None, forcing='guo', iso_filter=False):
         self.tenet = tenet
         self.xi_name, self.e_name = xi_name, e_name
         self.dims = tuple(int(d) for d in dims)
@@ -45,8 +45,15 @@ class SCEngine:
         d.tau = float(tau)
         if forcing not in ('guo', 'sc') or (forcing == 'sc' and mrt is not None):
             raise ValueError("forcing is 'guo', or 'sc' (Shan-Chen forcing, BGK only)")
+        if iso_filter and (forcing != 'guo' or mrt is not None or xi_name != 'D2Q9'
+                           or precision != 'fp64'):
+            raise ValueError('the isotropy filter is defined for D2Q9 BGK with the Guo '
+                             'forcing in FP64')
         self.forcing = forcing
+        self.iso_filter = bool(iso_filter)
         d.collision = (0 if forcing == 'guo' else 2) if mrt is None else 1
+        if iso_filter:
+            d.collision = 3
         d.precision = PRECISIONS[precision]
         d.rho0 = self.rho0
         keep = []

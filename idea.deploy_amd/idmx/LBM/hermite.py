@@ -139,6 +139,27 @@ def _float_bases(name):
     return f(Mw), f(_inv(Mw)), f(Mp), f(_inv(Mp))
 
 
+@lru_cache(maxsize=None)
+def iso_filter_tables():
+    """(M, Ai) float64 9 x 9 tables of the D2Q9 isotropy filter
+    (K_IsotropyFilter, idpy/LBM/LBMKernelsMeta.py:895-1080). M holds the
+    moment rows M' (1, x, y, xx, xy, yy, xxy, xyy, xxyy); the filter keeps
+    the moments 0-3 and 8 of the normalised populations and sets the raw
+    sums of xi_x xi_y, xi_x^2 - xi_y^2, xi_x^2 xi_y and xi_x xi_y^2 to values
+    computed from the others, so the new populations are Ai b with Ai the
+    inverse of the matrix of those nine rows."""
+    Mp, tuples = invertible_set('D2Q9')
+    if tuples != ((), (0,), (1,), (0, 0), (0, 1), (1, 1), (0, 0, 1), (0, 1, 1),
+                  (0, 0, 1, 1)):
+        raise RuntimeError('unexpected D2Q9 Hermite set %r' % (tuples,))
+    xis, _, _ = _lattice('D2Q9')
+    raw = lambda fn: tuple(Fr(fn(v[0], v[1])) for v in xis)
+    rows = list(Mp[:4]) + [raw(lambda x, y: x * y), raw(lambda x, y: x * x - y * y),
+                           raw(lambda x, y: x * x * y), raw(lambda x, y: x * y * y), Mp[8]]
+    f = lambda m: np.array([[float(x) for x in r] for r in m], dtype=np.float64)
+    return f(Mp), f(_inv(rows))
+
+
 def basis_vectors(name):
     """v_k = w * {1, xi_a, xi_a xi_b (a <= b)} as an NB x Q float64 array, in
     the order the kernel uses: 2D {1, x, y, xx, yy, xy}; 3D {1, x, y, z, xx,

@@ -59,8 +59,12 @@ enum {
  * u written as the mean of the pre- and post-collision velocities;
  * K_Collision_ShanChenMultiPhase[Walls], idpy/LBM/MultiPhaseKernels.py:
  * 132-304, used by MainLoopSimpleSC / MainLoopSimpleWallsSC), which runs
- * on the two-pass step. */
-enum { IDMX_BGK = 0, IDMX_MRT = 1, IDMX_BGK_SCF = 2 };
+ * on the two-pass step. IDMX_BGK_ISO (D2Q9, FP64 only) is BGK with the Guo
+ * forcing followed, on every site, by the isotropy filter of the
+ * post-collision populations (K_IsotropyFilter, idpy/LBM/LBMKernelsMeta.py:
+ * 895-1080, used by MainLoopIsoFilter, idpy/LBM/MultiPhase.py:1099-1223),
+ * also on the two-pass step. */
+enum { IDMX_BGK = 0, IDMX_MRT = 1, IDMX_BGK_SCF = 2, IDMX_BGK_ISO = 3 };
 enum { IDMX_PREC_F64 = 0, IDMX_PREC_F32S = 1 };
 
 /* Flags of idmx_sc_run. */
@@ -84,7 +88,7 @@ typedef struct {
   int psi_kind;       /* IDMX_PSI_*                                    */
   double SC_G;        /* coupling G                                    */
   double tau;         /* BGK relaxation time; OMEGA = 1/tau            */
-  int collision;      /* IDMX_BGK | IDMX_MRT | IDMX_BGK_SCF            */
+  int collision;      /* IDMX_BGK | IDMX_MRT | IDMX_BGK_SCF | IDMX_BGK_ISO */
   int precision;      /* IDMX_PREC_F64 | IDMX_PREC_F32S                */
   double rho0;        /* population shift for IDMX_PREC_F32S           */
   /* MRT only (collision == IDMX_MRT), row-major float64 tables built on

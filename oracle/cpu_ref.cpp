@@ -35,6 +35,9 @@
 //     and a body force F[1] += g; MainLoopSimpleSC / MainLoopSimpleWallsSC
 //     (MultiPhase.py:257-349, 467-581) with the Shan-Chen forcing collision
 //     (K_Collision_ShanChenMultiPhase[Walls], MultiPhaseKernels.py:132-304);
+//   * MainLoopIsoFilter (idpy/LBM/MultiPhase.py:1099-1223): the D2Q9 step
+//     with K_IsotropyFilter (idpy/LBM/LBMKernelsMeta.py:895-1080) between
+//     the collision and the push stream;
 //   * ComputeMomentsMeta (idpy/LBM/MultiPhase.py:1565-1662,
 //     K_ComputeVelocityAfterForceSCMPMeta MultiPhaseKernelsMeta.py:1077-1219)
 //   * InitPopulations (K_InitPopulations idpy/LBM/LBM.py:1700-1743)
@@ -349,10 +352,50 @@ int opposite(int q) {
   return q;
 }
 
+// K_IsotropyFilter (idpy/LBM/LBMKernelsMeta.py:895-1080) on the nine
+// post-collision populations of one D2Q9 site, in place: Hermite moments
+// m_k = sum_q M[k][q] f_q normalised by m_0 (:934-956), the xy, xx - yy,
+// xxy and xyy moments replaced by their values in terms of the others
+// (:962-1015), the populations solved back (f = Ai b, the inverse of the
+// system's matrix; the reference expands the symbolic solution instead) and
+// rescaled by m_0 (:1062-1072). M and Ai come from oracle/iso_filter.py.
+struct IsoMats {
+  const double* M;   // 9 x 9 moment rows (1, x, y, xx, xy, yy, xxy, xyy, xxyy)
+  const double* Ai;  // 9 x 9 inverse of the filter's system
+};
+inline void iso_filter_site(const IsoMats& t, double* f) {
+  double m[9];
+  for (int k = 0; k < 9; k++) {
+    double a = 0.;
+    for (int q = 0; q < 9; q++) a += t.M[k * 9 + q] * f[q];
+    m[k] = a;
+  }
+  for (int k = 1; k < 9; k++) m[k] /= m[0];
+  const double b[9] = {1.,
+                       m[1],
+                       m[2],
+                       m[3],
+                       m[1] * m[2],
+                       m[1] * m[1] - m[2] * m[2],
+                       2. * m[4] * m[1] + (m[3] + 1. / 3) * m[2] - 2. * m[2] * m[1] * m[1],
+                       2. * m[4] * m[2] + (m[5] + 1. / 3) * m[1] - 2. * m[1] * m[2] * m[2],
+                       m[8]};
+  for (int q = 0; q < 9; q++) {
+    double a = 0.;
+    for (int k = 0; k < 9; k++) a += t.Ai[q * 9 + k] * b[k];
+    f[q] = a * m[0];
+  }
+}
+
+// With iso (D2Q9, BGK) the step of MainLoopIsoFilter (MultiPhase.py:
+// 1099-1223): K_ForceCollideSCMPMeta collides in place, K_IsotropyFilter
+// filters, K_StreamPeriodicMeta push-streams -- the same as filtering f*
+// between the collision and the push of the fused step.
 template <class S>
 void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
                double omega, int psi_kind, const MRTMats* mrt, double*& pop,
-               double*& pop_swap, double* n, double* psi, const uint8_t* walls) {
+               double*& pop_swap, double* n, double* psi, const uint8_t* walls,
+               const IsoMats* iso = nullptr) {
   constexpr int D = S::D, Q = S::Q;
   const int64_t V = geo.V;
   // K_ComputeDensityPsiMeta (MultiPhaseKernelsMeta.py:107-167); with walls
@@ -400,6 +443,8 @@ void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
         fout[q] = a;
       }
     }
+    if constexpr (S::Q == 9)
+      if (iso) iso_filter_site(*iso, fout);
     // push stream: pop_swap[x + xi_q, q] = f*_q (Collisions.py:487-530);
     // with walls, a destination that is a solid site sends f*_q back to x as
     // the opposite population (Collisions.py:835-850)
@@ -563,6 +608,30 @@ int cpuref_meta_run(int xi_id, const int* L, int QE, const int* E,
     else meta_step<SD3Q19>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi, walls);
   }
   if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
+  return 0;
+}
+
+// nsteps of MainLoopIsoFilter (D2Q9, BGK, push): the Meta step with the
+// isotropy filter between collision and stream. M, Ai: oracle/iso_filter.py.
+int cpuref_iso_run(const int* L, int QE, const int* E, const double* EW, double G,
+                   double omega, int psi_kind, int aos, const double* M, const double* Ai,
+                   double* pop, double* pop_swap, double* n, double* psi, int nsteps) {
+  Geo geo;
+  if (!make_geo(2, L, &geo)) return -1;
+  EStencil es{QE, E, EW};
+  Ord ord{aos != 0, geo.V, 9};
+  IsoMats t{M, Ai};
+  double *a = pop, *b = pop_swap;
+  for (int s = 0; s < nsteps; s++)
+    meta_step<SD2Q9>(geo, es, ord, G, omega, psi_kind, nullptr, a, b, n, psi, nullptr, &t);
+  if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * 9);
+  return 0;
+}
+
+// The filter alone on nsites independent sites, f[q + 9 s], in place.
+int cpuref_iso_filter(int64_t nsites, const double* M, const double* Ai, double* f) {
+  IsoMats t{M, Ai};
+  for (int64_t s = 0; s < nsites; s++) iso_filter_site(t, f + 9 * s);
   return 0;
 }
 

@@ -36,6 +36,10 @@ def lib():
         L.cpuref_legacy_walls_run.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
                                               C.c_double, C.c_int, C.c_void_p, C.c_double,
                                               C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int]
+        L.cpuref_iso_run.argtypes = [_ip, C.c_int, _ip, _dp, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     C.c_int]
+        L.cpuref_iso_filter.argtypes = [C.c_int64, _dp, _dp, _dp]
         L.cpuref_init_pop.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp]
         L.cpuref_init_radial.argtypes = [C.c_int, _ip, C.c_double, C.c_double,
                                          C.c_double, C.c_double, _dp, _dp]
@@ -157,3 +161,31 @@ def meta_moments(st, estencil, G, psi='exp', walls=None):
                                    PSI_KIND[psi], int(st.aos), st.pop, st.n, st.u, st.psi,
                                    wptr)
     assert rc == 0
+
+
+def _iso_tables():
+    from . import iso_filter
+    M, Ai = iso_filter.tables()
+    return np.ascontiguousarray(M), np.ascontiguousarray(Ai)
+
+
+def iso_run(st, estencil, G, tau, psi='exp', nsteps=1):
+    """MainLoopIsoFilter (idpy/LBM/MultiPhase.py:1099-1223), D2Q9 BGK:
+    the Meta step with K_IsotropyFilter between collision and push."""
+    assert st.xi == 'D2Q9'
+    QE, Ei, Ew = _estencil(estencil)
+    M, Ai = _iso_tables()
+    rc = lib().cpuref_iso_run(_L(st.dims), QE, Ei, Ew, G, 1.0 / tau, PSI_KIND[psi],
+                              int(st.aos), M, Ai, st.pop, st.pop_swap, st.n, st.psi, nsteps)
+    assert rc == 0
+
+
+def iso_filter(f):
+    """K_IsotropyFilter (idpy/LBM/LBMKernelsMeta.py:895-1080) on an
+    (nsites, 9) array of populations; returns the filtered copy."""
+    out = np.ascontiguousarray(f, dtype=np.float64).copy()
+    assert out.ndim == 2 and out.shape[1] == 9
+    M, Ai = _iso_tables()
+    rc = lib().cpuref_iso_filter(out.shape[0], M, Ai, out)
+    assert rc == 0
+    return out

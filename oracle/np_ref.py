@@ -52,9 +52,38 @@ def feq_guo(xi, n, u, F):
     return np.array(feq), np.array(fg)
 
 
-def step(f, xi, estencil, G, tau, psi_kind='exp', mrt=None):
+def iso_filter_solve(f):
+    """K_IsotropyFilter (idpy/LBM/LBMKernelsMeta.py:895-1080) by a dense
+    solve at every site: f has shape (9, ...). Moment rows are built here
+    from the lattice vectors (Hermite products h_a(xi_x) h_b(xi_y) with
+    h_0 = 1, h_1 = x, h_2 = x^2 - 1/3), normalised by the density; the new
+    populations solve the system whose xy, xx - yy, xxy and xyy rows take
+    the reference's substituted values."""
+    x = np.array([v[0] for v in XI['D2Q9']], dtype=np.float64)
+    y = np.array([v[1] for v in XI['D2Q9']], dtype=np.float64)
+    h2 = lambda a: a * a - 1.0 / 3.0
+    one = np.ones(9)
+    mrows = np.array([one, x, y, h2(x), x * y, h2(y), h2(x) * y, x * h2(y), h2(x) * h2(y)])
+    arows = np.array([one, x, y, h2(x), x * y, x * x - y * y, x * x * y, x * y * y,
+                      h2(x) * h2(y)])
+    sh = f.shape[1:]
+    ff = f.reshape(9, -1)
+    m = mrows @ ff
+    m0 = m[0].copy()
+    m[1:] /= m0
+    t = 1.0 / 3.0
+    b = np.array([np.ones_like(m0), m[1], m[2], m[3], m[1] * m[2], m[1] ** 2 - m[2] ** 2,
+                  2 * m[4] * m[1] + (m[3] + t) * m[2] - 2 * m[2] * m[1] ** 2,
+                  2 * m[4] * m[2] + (m[5] + t) * m[1] - 2 * m[1] * m[2] ** 2, m[8]])
+    out = np.linalg.solve(arows, b) * m0
+    return out.reshape((9,) + sh)
+
+
+def step(f, xi, estencil, G, tau, psi_kind='exp', mrt=None, iso=False):
     """One step: density/psi, force, Guo-shifted velocity, BGK (or the
-    dense MRT form A f + B f_eq + Gg F_guo), push streaming. Returns
+    dense MRT form A f + B f_eq + Gg F_guo), push streaming; with iso
+    (D2Q9) the isotropy filter between collision and stream, the step of
+    MainLoopIsoFilter (idpy/LBM/MultiPhase.py:1099-1223). Returns
     (f_new, n, psi, u_shifted)."""
     omega = 1.0 / tau
     xis = XI[xi]
@@ -70,6 +99,8 @@ def step(f, xi, estencil, G, tau, psi_kind='exp', mrt=None):
         A, B, Gg = mrt
         fs = (np.tensordot(A, f, axes=1) + np.tensordot(B, feq, axes=1)
               + np.tensordot(Gg, fg, axes=1))
+    if iso:
+        fs = iso_filter_solve(fs)
     fn = np.empty_like(f)
     for q, v in enumerate(xis):
         fn[q] = _shift(fs[q], tuple(-c for c in v))
