@@ -125,6 +125,15 @@ class ShanChenMultiPhase(_ShanChenCommon):
         with eng.configured(walls=m['walls']):
             return self._run_loop(time_steps, convergence_functions, profiling, eng, m['u'])
 
+    def MainLoopSimple(self, time_steps, convergence_functions=[], profiling=False):
+        """MainLoopSimple (MultiPhase.py:164-255): the legacy-kernel loop
+        (K_ComputeMoments, K_ComputePsi, K_Collision_ShanChenGuoMultiPhase,
+        K_StreamPeriodic, LBM.py:1529-1698) on the periodic lattice: the same
+        update as MainLoop, with n, psi and the Guo-shifted u of the last step
+        written, as the legacy kernels leave them."""
+        return self._run_loop(time_steps, convergence_functions, profiling, self.engine,
+                              self.sims_idpy_memory['u'])
+
     def MainLoopSimpleWalls(self, time_steps, convergence_functions=[], profiling=False):
         """MainLoopSimpleWalls (MultiPhase.py:351-465): the legacy-kernel walls
         loop (K_ComputeMomentsWalls, K_ComputePsiWalls,

@@ -267,3 +267,33 @@ def test_wall_loops_fp32_shifted_close_to_fp64_oracle(tenet, loop):
     print(loop, 'fp32s rel err', err, 'mass drift', abs(m1 - m0) / m0, flush=True)
     assert err <= 2e-5
     assert abs(m1 - m0) <= 1e-5 * m0
+
+
+@pytest.mark.parametrize('xi,es,dims,nsteps', [('D2Q9', 'D2E4', (64, 37), 200),
+                                               ('D3Q19', 'D3E4', (21, 18, 13), 40)])
+def test_simple_loop_matches_oracle_legacy_kernels(tenet, xi, es, dims, nsteps):
+    """MainLoopSimple (MultiPhase.py:164-255, the legacy kernels on the
+    periodic lattice) against the oracle's legacy loop: populations, n, psi
+    and the Guo-shifted u it writes."""
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    from oracle import cpuref
+    n0 = droplet_field(dims, seed=49)
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=dims, xi_stencil=XIStencils[xi],
+                             f_stencil=FStencils[es], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=G, tau=1.0)
+    sim.SetDensityField(n0)
+    sim.MainLoopSimple([0, nsteps // 2, nsteps])
+    st = cpuref.State(xi, dims)
+    cpuref.init_pop(st, n0.ravel(), np.zeros(st.V * st.D))
+    cpuref.legacy_walls_run(st, es, G, 1.0, None, nsteps=nsteps)
+    m = sim.sims_idpy_memory
+    e_pop = rel_err(m['pop'].D2H().ravel(), st.pop)
+    e_n = rel_err(m['n'].D2H(), st.n)
+    e_psi = rel_err(m['psi'].D2H(), st.psi)
+    e_u = rel_err(m['u'].D2H().ravel(), st.u)
+    sim.End()
+    print('rel err pop', e_pop, 'n', e_n, 'psi', e_psi, 'u', e_u, flush=True)
+    assert e_pop <= 1e-12 and e_n <= 1e-12 and e_psi <= 1e-12
+    assert e_u <= 1e-11
