@@ -134,6 +134,38 @@ class ShanChenMultiPhase(_ShanChenCommon):
         return self._run_loop(time_steps, convergence_functions, profiling, self.engine,
                               self.sims_idpy_memory['u'])
 
+    def MainLoopSimpleWallsInletOutlet(self, time_steps, convergence_functions=[],
+                                       profiling=False, n_in=1, u_in=0, tau_in=100, max_mult=2):
+        """Not provided. MultiPhase.py:583-733 deploys K_SetPopNUXInletDutyCycle
+        with 7 arguments (:659) against the 9 parameters of its signature
+        (LBMKernels.py:307-315), and the kernel sets n = 2 whatever n_in is
+        (LBMKernels.py:330), so the reference defines no behaviour to match."""
+        raise NotImplementedError('MainLoopSimpleWallsInletOutlet: the reference loop passes 7 '
+                                  'arguments to a 9-parameter inlet kernel; not provided')
+
+    def MainLoopSimpleWallsInletOutletSC(self, time_steps, convergence_functions=[],
+                                         profiling=False, n_in=1, u_in=0, tau_in=100,
+                                         max_mult=2):
+        """Not provided, for the reason given in MainLoopSimpleWallsInletOutlet
+        (the same inlet kernel call, MultiPhase.py:735-886)."""
+        raise NotImplementedError('MainLoopSimpleWallsInletOutletSC: the reference loop passes 7 '
+                                  'arguments to a 9-parameter inlet kernel; not provided')
+
+    def MainLoopGross2011SRT(self, *args, **kwargs):
+        """Not provided: fluctuating LBM (MultiPhase.py:1404-1541) is outside
+        the deterministic step this library implements."""
+        raise NotImplementedError('MainLoopGross2011SRT: fluctuating LBM is not supported')
+
+    def ComputeMoments(self):
+        """ComputeMoments (MultiPhase.py:1543-1563): K_ComputeMoments on the
+        device, n = sum f and u = sum xi f / n (no force shift)."""
+        if not self.init_status['pop']:
+            raise Exception('Populations are not initialized')
+        m = self.sims_idpy_memory
+        self.engine.density_velocity(m['pop'], m['n'], m['u'])
+        self.init_status['n'] = True
+        self.init_status['u'] = True
+
     def MainLoopSimpleWalls(self, time_steps, convergence_functions=[], profiling=False):
         """MainLoopSimpleWalls (MultiPhase.py:351-465): the legacy-kernel walls
         loop (K_ComputeMomentsWalls, K_ComputePsiWalls,
