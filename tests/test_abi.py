@@ -39,3 +39,26 @@ def test_error_channel_without_gpu():
     rc = L.idmx_sc_create(None, None, None)
     assert rc == -1
     assert 'idmx_sc_create' in _lib.last_error()
+
+
+def test_product_package_never_reaches_the_oracle():
+    """The oracle is test infrastructure: no module of the product package
+    imports it, loads its library or names its folder."""
+    pkg = os.path.join(ROOT, 'idea.deploy_amd')
+    hits = []
+    for d, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith(('.py', '.h', '.hip', '.cpp')):
+                txt = open(os.path.join(d, f), encoding='utf-8').read()
+                if re.search(r'\bimport\s+oracle\b|\bfrom\s+oracle\b|libcpuref|cpuref\.', txt):
+                    hits.append(os.path.join(d, f))
+    assert not hits, hits
+
+
+def test_collision_enum_values():
+    """IDMX_BGK / _MRT / _BGK_SCF / _BGK_ISO keep the values the Python
+    engine writes into idmx_sc_desc.collision (0, 1, 2, 3)."""
+    txt = open(os.path.join(ROOT, 'include', 'idmx.h')).read()
+    vals = dict((k, int(v)) for k, v in re.findall(r'(IDMX_(?:BGK|MRT|BGK_SCF|BGK_ISO))\s*=\s*(\d+)',
+                                                    txt))
+    assert vals == {'IDMX_BGK': 0, 'IDMX_MRT': 1, 'IDMX_BGK_SCF': 2, 'IDMX_BGK_ISO': 3}
