@@ -70,7 +70,7 @@ def workload_desc(name, w):
     return d
 
 
-def run_gpu(name, steps, warmup, tune=True, launch=None):
+def run_gpu(name, steps, warmup, tune=True, launch=None, two_pass=False):
     from idmx import configs
     from idmx.IdpyCode import IdpyMemory, Tenet
     from idmx.LBM import hermite
@@ -86,21 +86,23 @@ def run_gpu(name, steps, warmup, tune=True, launch=None):
     V, D = eng.V, eng.D
     if launch:
         eng.set_launch(*launch)
-    elif tune:
+    elif tune and not two_pass:
         from idmx import tune as tuner
         tuner.tune_sim(eng, steps=5, use_cache=False)
     n = IdpyMemory.OnDevice(n0, tenet)
     u = IdpyMemory.Zeros(V * D, np.float64, tenet)
+    psi = IdpyMemory.Zeros(V, np.float64, tenet) if two_pass else None
     pop = PopArray(eng, tenet)
     swap = PopArray(eng, tenet)
     eng.init_pop(pop, n, u)
     del n0
     m0, _ = eng.mass_momentum(pop)
-    if warmup > 0 and eng.run(pop, swap, warmup):
+    kw = {'n': n, 'psi': psi, 'two_pass': True} if two_pass else {}
+    if warmup > 0 and eng.run(pop, swap, warmup, **kw):
         pop, swap = swap, pop
     tenet.Sync()
     t0 = time.perf_counter()
-    if eng.run(pop, swap, steps):
+    if eng.run(pop, swap, steps, **kw):
         pop, swap = swap, pop
     tenet.Sync()
     wall = time.perf_counter() - t0
@@ -441,6 +443,9 @@ def main():
     ap.add_argument('--precision-study', type=int, default=0, metavar='STEPS',
                     help='FP64 vs FP32-shifted observables on the D3Q19 MRT 256^3 droplet '
                          'after STEPS steps')
+    ap.add_argument('--two-pass-study', action='store_true',
+                    help='time the two-kernel step (the reference kernel structure) on each '
+                         'workload instead of the step benchmark')
     ap.add_argument('--sweep', action='store_true',
                     help='run the (G, R, L) sweep study (time to steady state, learned '
                          'warm start vs plain) instead of the step benchmark')
@@ -467,6 +472,24 @@ def main():
         return
     if args.tune_report:
         tune_report(args.tune_report.split(','))
+        return
+    if args.two_pass_study:
+        # the reference's kernel structure (density/psi pass, then force-
+        # collide-stream) on the GPU: 3 Q s + 32 bytes per site (n and psi
+        # written, populations read twice, n and psi read back), against the
+        # fused step's 2 Q s
+        out = {}
+        for x in [args.workload] + [e for e in args.extra.split(',') if e and e != args.workload]:
+            rx = run_gpu(x, 50, 5, tune=False, two_pass=True)
+            s = rx['B_alg'] // (2 * rx['Q'])
+            ref_b = 3 * rx['Q'] * s + 32
+            out[x] = {'mlups': round(rx['mlups'], 1), 'kernel_ms': round(rx['kernel_ms'], 4),
+                      'frac_of_fused_bytes': round(rx['B_alg'] * rx['V'] / (rx['kernel_ms'] / 1e3)
+                                                   / 1e9 / PEAK_HBM_GBS, 4),
+                      'structure_bytes_per_site': ref_b,
+                      'frac_of_structure_bytes': round(ref_b * rx['V'] / (rx['kernel_ms'] / 1e3)
+                                                       / 1e9 / PEAK_HBM_GBS, 4)}
+        print(json.dumps({'two_pass_study': out}), flush=True)
         return
     if args.sweep_batch_only:
         r = run_sweep_batch_only(args.sweep_batch_only, args.sweep_Lmax, args.sweep_mode)
