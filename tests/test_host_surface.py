@@ -1,0 +1,34 @@
+"""Host-side surface of the drop-in classes without a GPU: RootLB's sizes,
+strides, V and their dtypes (the checks of the reference's own
+test_RootLBVars, idpy/LBM/test.py:318-362), and the lexicographic index
+helpers IndexFromPos / PosFromIndex (idpy/LBM/MultiPhase.py:94-106)."""
+import numpy as np
+import pytest
+
+
+@pytest.mark.parametrize('dims,xi', [((11, 7), 'D2Q9'), ((5, 6, 7), 'D3Q19')])
+def test_rootlb_sizes_strides_and_volume(dims, xi):
+    from idmx.LBM.LBM import LBMTypes, RootLB, XIStencils
+    r = RootLB(dim_sizes=dims, xi_stencil=XIStencils[xi], custom_types=LBMTypes)
+    v = r.sims_vars
+    assert list(v['dim_sizes']) == list(dims)
+    assert v['dim_sizes'].dtype == np.int32 and v['dim_strides'].dtype == np.int32
+    strides = [int(np.prod(dims[:k + 1])) for k in range(len(dims) - 1)]
+    assert list(v['dim_strides']) == strides
+    assert v['V'] == int(np.prod(dims))
+    assert v['Q'] == (9 if xi == 'D2Q9' else 19) and v['DIM'] == len(dims)
+
+
+@pytest.mark.parametrize('dims', [(11, 7), (5, 6, 7)])
+def test_index_position_round_trip(dims):
+    """Every site: PosFromIndex inverts IndexFromPos, and the index is
+    x + L0 (y + L1 z), x fastest."""
+    from idmx.LBM.MultiPhase import IndexFromPos, PosFromIndex
+    strides = [int(np.prod(dims[:k + 1])) for k in range(len(dims) - 1)]
+    V = int(np.prod(dims))
+    for g in range(V):
+        pos = PosFromIndex(g, strides)
+        assert all(0 <= p < L for p, L in zip(pos, dims))
+        lex = pos[0] + dims[0] * (pos[1] + (dims[1] * pos[2] if len(dims) == 3 else 0))
+        assert lex == g
+        assert IndexFromPos(pos, strides) == g
