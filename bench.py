@@ -537,9 +537,13 @@ def main():
         rx = run_gpu(x, max(20, args.steps // 4), max(5, args.warmup // 4),
                      tune=not args.no_tune)
         ax = rx['B_alg'] * rx['V'] / (rx['kernel_ms'] / 1e3) / 1e9
+        tx, tlx = traffic_from_profiles(x)
         extra[x] = {'mlups': round(rx['mlups'], 1), 'hbm_frac': round(ax / PEAK_HBM_GBS, 4),
                     'achieved_GBs': round(ax, 1), 'kernel_ms': round(rx['kernel_ms'], 4),
-                    'mass_rel_drift': rx['mass_rel_drift']}
+                    'mass_rel_drift': rx['mass_rel_drift'], 'traffic': tx,
+                    'traffic_launch': tlx,
+                    'launch': [rx['launch'][k] for k in ('tile_x', 'tile_y',
+                                                         'planes_per_block', 'variant')]}
     if extra:
         line['extra'] = extra
     if rank == 0 and not args.no_cpu:
