@@ -459,7 +459,7 @@ def main():
     ap.add_argument('--sweep-epochs', type=int, default=120)
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU: this build has no multi-GPU path '
+        print('bench.py measures one GPU: the library has no multi-GPU path '
               '(see the sharding design); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get('RANK', '0'))
