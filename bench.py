@@ -460,7 +460,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: the library has no multi-GPU path '
-              '(see the sharding design); run with --gpus 1', file=sys.stderr)
+              '(a lattice would be cut into slabs along its slowest axis, one per GPU); '
+              'run with --gpus 1', file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get('RANK', '0'))
     if args.calibrate:
