@@ -110,10 +110,16 @@ def tune_batch(tenet, points, precision, estencil='E4', steps=20, use_cache=True
     timings = {}
     for cand in batch_candidates(cus):
         b.set_launch(*cand)
-        b.run(2)
-        t_short = b.run(2)
-        # difference of two run lengths: the pull form's conversions cancel
-        timings[','.join(map(str, cand))] = max(b.run(2 + steps) - t_short, 1e-6) / steps
+        # each length runs twice and the second run is timed: the first
+        # captures the step graph for that length, whose capture and
+        # instantiation (tens of microseconds, more than a small batch's
+        # step) are not step time; the difference of the two lengths
+        # cancels the pull form's conversions
+        b.run(steps)
+        t_a = b.run(steps)
+        b.run(2 * steps)
+        t_b = b.run(2 * steps)
+        timings[','.join(map(str, cand))] = max(t_b - t_a, 1e-6) / steps
     b.close()
     best = min(timings, key=timings.get)
     best = tuple(int(v) for v in best.split(','))
@@ -121,10 +127,16 @@ def tune_batch(tenet, points, precision, estencil='E4', steps=20, use_cache=True
     return best, timings
 
 
-def tune_batch_small(tenet, points, precision, estencil='E4', count=3, steps=20):
-    """The launch for the end of a sweep, when only a few domains are left:
-    tuned on the `count` largest domains of `points`. Returns ((kind, tx, H,
-    grid), their total sites), the form run_sweep's launch_small takes."""
+def tune_batch_small(tenet, points, precision, estencil='E4', count=None, steps=20):
+    """The launch for the second part of a sweep, once the live domains hold
+    no more sites than the `count` largest domains of `points` (default: the
+    larger half, at least 3), tuned on those domains. Returns ((kind, tx, H,
+    grid), their total sites), the form run_sweep's launch_small takes.
+    Measured on a 128-point sweep: switching at the larger half instead of
+    the 3 largest cut the batched wall time from 10.1 s to 8.8 s (the tile
+    launches beat the marching ones once a few dozen domains are left)."""
+    if count is None:
+        count = max(3, len(points) // 2)
     big = sorted(points, key=lambda p: p.L, reverse=True)[:count]
     best, _ = tune_batch(tenet, big, precision, estencil, steps=steps, use_cache=False)
     return best, sum(p.L * p.L for p in big)
