@@ -51,6 +51,42 @@ def test_batch_matches_single_domain(tenet, precision, tol):
     b.close()
 
 
+def test_batch_one_item_per_workgroup(tenet):
+    """Grids of one work item per workgroup and beyond (the autotuner's
+    largest candidates; surplus workgroups exit at once), and the same grid
+    after domains leave the live set, when it exceeds the item count."""
+    from idmx.sweep import Batch, cold_start
+    pts = _points()
+    b = Batch(tenet, pts, 'fp64')
+    starts = [cold_start(p) + 1e-4 * np.cos(np.arange(p.L * p.L)) for p in pts]
+    refs = [_single(tenet, p, starts[i], 150, 'fp64') for i, p in enumerate(pts)]
+    for kind, tx, H in [(3, 32, 8), (1, 64, 4), (2, 128, 8)]:
+        b.set_launch(kind, tx, H)
+        items = b.get_launch()[4]
+        assert items > 0
+        for grid in (items, 4 * items):
+            print('launch shape', (kind, tx, H, grid), flush=True)
+            b.set_launch(kind, tx, H, grid)
+            for i, n0 in enumerate(starts):
+                b.set_density(i, n0)
+            b.run(150)
+            for i, p in enumerate(pts):
+                assert rel_err(b.get_density(i).ravel(), refs[i]) <= 1e-13, (kind, grid, i)
+    b.set_launch(3, 32, 8)
+    full = b.get_launch()[4]
+    b.set_launch(3, 32, 8, full)
+    for i, n0 in enumerate(starts):
+        b.set_density(i, n0)
+    b.run(100)
+    b.set_active([0, 1, 0, 1])
+    kind, tx, H, grid, items = b.get_launch()
+    assert grid == full and 0 < items < full
+    b.run(50)
+    for i in (1, 3):
+        assert rel_err(b.get_density(i).ravel(), refs[i]) <= 1e-13, i
+    b.close()
+
+
 def test_batch_check_and_active_set(tenet):
     from idmx.sweep import Batch, cold_start
     pts = _points()
