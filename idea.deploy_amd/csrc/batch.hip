@@ -76,6 +76,8 @@ int rebuild_items(idmx_batch_s* b) {
     drop_graph(b);  // the graph holds the old item pointer
   }
   b->n_items = (int)items.size();
+  // the graph's step launches hold the grid they were captured with
+  if (b->graph && step_grid(b) != b->graph_grid) drop_graph(b);
   if (!items.empty())
     IDMX_HIP(hipMemcpyAsync(b->d_items, items.data(), sizeof(BatchItem) * items.size(),
                             hipMemcpyHostToDevice, b->ctx->stream));
@@ -285,6 +287,7 @@ int idmx_batch_run(idmx_batch_t b, int nsteps) {
       IDMX_HIP(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
       hipGraphDestroy(g);
       b->graph_steps = G;
+      b->graph_grid = step_grid(b);
     }
     int done = 0;
     // the graph was captured for the parity the state had then; cur is

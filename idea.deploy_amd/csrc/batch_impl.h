@@ -6,6 +6,7 @@
 // trajectory.
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "batch_kernels.h"
@@ -35,14 +36,22 @@ struct idmx_batch_s {
   int grid = 0;
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
+  int graph_grid = 0;  // step grid the graph was captured with
   double last_ms = 0;
 };
 
 namespace idmx {
 
+// Blocks of a step launch: the tuned grid, but never more than one per live
+// work item. The step kernels order their items per XCD (xcd_block), which
+// spreads the items over all XCDs only when no block is surplus.
+inline int step_grid(const idmx_batch_s* b) {
+  return std::max(1, std::min(b->grid, b->n_items));
+}
+
 template <class ES, class T, bool SHIFT, int TX, bool PULL>
 hipError_t launch_march(idmx_batch_s* b, const void* in, void* out) {
-  hipLaunchKernelGGL((k_batched<ES, T, SHIFT, TX, PULL>), dim3(b->grid), dim3(TX), 0,
+  hipLaunchKernelGGL((k_batched<ES, T, SHIFT, TX, PULL>), dim3(step_grid(b)), dim3(TX), 0,
                      b->ctx->stream, (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems);
   return hipGetLastError();
 }
@@ -56,11 +65,11 @@ hipError_t launch_step(idmx_batch_s* b, const void* in, void* out) {
 template <class ES, class T, bool SHIFT, int BX, int BY>
 hipError_t launch_tile(idmx_batch_s* b, const void* in, void* out) {
   if (b->kind == 3)
-    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, true>), dim3(b->grid),
+    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, true>), dim3(step_grid(b)),
                        dim3(BX * BY), 0, b->ctx->stream, (const T*)in, (T*)out, b->d_doms,
                        b->d_items, b->d_nitems);
   else
-    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, false>), dim3(b->grid),
+    hipLaunchKernelGGL((k_batched_tile<ES, T, SHIFT, BX, BY, false>), dim3(step_grid(b)),
                        dim3(BX * BY), 0, b->ctx->stream, (const T*)in, (T*)out, b->d_doms,
                        b->d_items, b->d_nitems);
   return hipGetLastError();

@@ -115,6 +115,17 @@ __device__ inline Phys<T> dom_phys(const BatchDom& d) {
   return P;
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8),
+// each with its own L2. This permutation of [0, G) hands XCD x the
+// contiguous virtual indices [x q + min(x, r), ...), q + (x < r) of them
+// (q = G / 8, r = G % 8), so that neighbouring work items, which re-read
+// each other's edge populations, mostly run on one XCD. Any grid size works.
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, G = gridDim.x;
+  const int x = b & 7, q = G >> 3, r = G & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 template <class ES, class T, bool SHIFT, int TX, bool PULL = false>
 __global__ void __launch_bounds__(TX)
 k_batched(const T* __restrict__ pop_all, T* __restrict__ swap_all,
@@ -122,7 +133,7 @@ k_batched(const T* __restrict__ pop_all, T* __restrict__ swap_all,
           const int* __restrict__ n_items) {
   __shared__ T lds[2 * ES::R + 2][TX + 2 * ES::R];
   const int ni = *n_items;
-  for (int it = blockIdx.x; it < ni; it += gridDim.x) {
+  for (int it = xcd_block(); it < ni; it += gridDim.x) {
     const BatchItem w = items[it];
     const BatchDom d = doms[w.dom];
     const Geom geo{d.L0, d.L1, 1, (int64_t)d.L0 * d.L1};
@@ -165,7 +176,7 @@ k_batched_tile(const T* __restrict__ pop_all, T* __restrict__ swap_all,
   __shared__ T lds[WY][WX];
   const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
   const int ni = *n_items;
-  for (int it = blockIdx.x; it < ni; it += gridDim.x) {
+  for (int it = xcd_block(); it < ni; it += gridDim.x) {
     const BatchItem w = items[it];
     const BatchDom d = doms[w.dom];
     const Geom geo{d.L0, d.L1, 1, (int64_t)d.L0 * d.L1};
