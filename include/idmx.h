@@ -287,8 +287,10 @@ int idmx_batch_destroy(idmx_batch_t b);
  * LDS (small domains: no dependent row loads). kind 2 / 3: kind 0 / 1 run on
  * the pull form (post-collision values at their own site, read from the
  * source site: aligned stores); idmx_batch_run converts every live domain to
- * that form before its steps and back after them. grid = persistent blocks
- * (0 keeps the current grid). Picked by the autotuner (idmx.tune). */
+ * that form before its steps and back after them. grid = blocks of a step
+ * launch (0 keeps the current grid); a step launches at most one block per
+ * live work item, and each XCD takes a contiguous run of items. Picked by
+ * the autotuner (idmx.tune). */
 int idmx_batch_set_launch(idmx_batch_t b, int kind, int tx, int H, int grid);
 int idmx_batch_get_launch(idmx_batch_t b, int* kind, int* tx, int* H, int* grid,
                           int* n_items);
