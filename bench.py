@@ -361,6 +361,41 @@ def run_sweep_batch_only(n_points, Lmax, mode='fp64', max_steps=400000):
     return out
 
 
+def run_batch_step_study(n_points, Lmax, steps, launch=None):
+    """The batched FP64 step of the sweep study with every domain live: its
+    n_points centred droplets (L uniform in {64, 96, ..., Lmax}, cold start),
+    the autotuner's launch (or `launch` = (kind, tx, H, grid)), time per step
+    from HIP events on the library's stream as the difference of a run of
+    2 * steps and a run of `steps` (each run twice, the second timed: the
+    first captures its graph; the difference cancels the pull form's
+    conversions). Algorithmic bytes 144 per site (nine FP64 populations read
+    once and written once)."""
+    from idmx import configs, sweep, tune
+    from idmx.IdpyCode import Tenet
+    tenet = Tenet(0)
+    pts = sweep.sample_points(n_points, configs.SEED0 + 4, Ls=tuple(range(64, Lmax + 1, 32)))
+    if launch is None:
+        launch, _ = tune.tune_batch(tenet, pts, 'fp64', use_cache=False)
+    b = sweep.Batch(tenet, pts, 'fp64')
+    for i, p in enumerate(pts):
+        b.set_density(i, sweep.cold_start(p))
+    b.set_launch(*launch)
+    b.run(steps)
+    t_a = b.run(steps)
+    b.run(2 * steps)
+    t_b = b.run(2 * steps)
+    ms = (t_b - t_a) / steps
+    sites = int(sum(p.L * p.L for p in pts))
+    gbs = 144 * sites / (ms / 1e3) / 1e9
+    out = {'n_points': n_points, 'L': [64, Lmax], 'sites': sites, 'launch': list(launch),
+           'work_items': b.get_launch()[4], 'steps_timed': steps, 'ms_per_step': ms,
+           'mlups': sites / (ms / 1e3) / 1e6, 'achieved_GBs': gbs,
+           'hbm_frac': gbs / PEAK_HBM_GBS, 'bytes_per_site': 144}
+    b.close()
+    tenet.End()
+    return out
+
+
 def calibrate(reps=10):
     """Plain copy of 19 x 256^3 float64 values (2.55 GB read + 2.55 GB
     written per launch): achievable HBM bandwidth and the reference launch
@@ -452,6 +487,11 @@ def main():
     ap.add_argument('--sweep-batch-only', type=int, default=0, metavar='N',
                     help='time the batched sweep alone on N points (--sweep-Lmax, '
                          '--sweep-mode)')
+    ap.add_argument('--batch-step-study', type=int, default=0, metavar='STEPS',
+                    help='time the batched FP64 step with every domain of the sweep '
+                         'study live (--sweep-eval points, --sweep-Lmax) over STEPS steps')
+    ap.add_argument('--batch-launch', default=None,
+                    help='kind,tx,H,grid for --batch-step-study (default: autotuned)')
     ap.add_argument('--sweep-mode', default='fp64', choices=['fp64', 'fp32+fp64'])
     ap.add_argument('--sweep-train', type=int, default=512)
     ap.add_argument('--sweep-eval', type=int, default=128)
@@ -491,6 +531,11 @@ def main():
                       'frac_of_structure_bytes': round(ref_b * rx['V'] / (rx['kernel_ms'] / 1e3)
                                                        / 1e9 / PEAK_HBM_GBS, 4)}
         print(json.dumps({'two_pass_study': out}), flush=True)
+        return
+    if args.batch_step_study:
+        bl = tuple(int(v) for v in args.batch_launch.split(',')) if args.batch_launch else None
+        r = run_batch_step_study(args.sweep_eval, args.sweep_Lmax, args.batch_step_study, bl)
+        print(json.dumps({'batch_step_study': r}), flush=True)
         return
     if args.sweep_batch_only:
         r = run_sweep_batch_only(args.sweep_batch_only, args.sweep_Lmax, args.sweep_mode)
