@@ -1,5 +1,6 @@
 """Host logic of the sweep driver that needs no GPU: sweep point sampling,
-the droplet end-state classifier and the Laplace pressure probe."""
+the droplet end-state classifier, the Laplace pressure probe and the batch
+autotuner's candidate launches."""
 import numpy as np
 
 
@@ -32,3 +33,18 @@ def test_laplace_probe_sign_and_value():
     n = cold_start(SweepPoint(L=64, G=G, R=12.0)).reshape(64, 64)
     p = lambda v: v / 3 + 0.5 * G * (1 - np.exp(-v)) ** 2
     assert abs(laplace_dp(n, G) - (p(n[32, 32]) - p(n[-1, -1]))) < 1e-15
+
+
+def test_batch_candidates_grids():
+    """Every shape with 4, 8 and 16 blocks per CU, plus one block per work
+    item where the batch has more items than the largest of those grids."""
+    from idmx import tune
+    shapes = tune.batch_shapes()
+    assert len(set(shapes)) == len(shapes) == 26
+    base = tune.batch_candidates(256)
+    assert len(base) == 3 * len(shapes)
+    assert {c[3] for c in base} == {1024, 2048, 4096}
+    items = {s: (5000 if s[0] in (1, 3) else 4096) for s in shapes}
+    cands = tune.batch_candidates(256, items)
+    extra = [c for c in cands if c not in base]
+    assert sorted(extra) == sorted(s + (5000,) for s in shapes if s[0] in (1, 3))
