@@ -258,41 +258,47 @@ int idmx_batch_get_density(idmx_batch_t b, int dom, double* n_host) {
 int idmx_batch_run(idmx_batch_t b, int nsteps) {
   BCHECK(b && nsteps >= 0, "idmx_batch_run: invalid argument");
   hipStream_t st = b->ctx->stream;
-  IDMX_HIP(hipEventRecord(b->ctx->ev0, st));
   const bool pull = (b->kind == 2 || b->kind == 3) && b->n_items > 0 && nsteps > 0;
+  // Inactive domains are never written by the step or conversion launches:
+  // their state stays in the buffer that was current on entry.
+  const int cur_in = b->cur;
+  // The graph (an even number of steps: A->B, B->A, ...; replays keep the
+  // state in the same buffer) is captured before the timed region, for the
+  // buffer parity the steps will see: flipped by the conversion to the
+  // pull form.
+  const int G = 2 * std::max(1, std::min(nsteps / 2, 50));
+  const int cur_steps = b->cur ^ (pull ? 1 : 0);
+  if (b->n_items > 0 && nsteps >= G && (b->graph_steps != G || b->graph_cur != cur_steps)) {
+    drop_graph(b);
+    hipGraph_t g;
+    IDMX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < G; k++) {
+      const int src = (cur_steps + k) & 1;
+      hipError_t e = one_step(b, b->pop[src], b->pop[src ^ 1]);
+      if (e != hipSuccess) {
+        hipStreamEndCapture(st, &g);
+        return hip_fail(e, "idmx_batch_run capture");
+      }
+    }
+    IDMX_HIP(hipStreamEndCapture(st, &g));
+    IDMX_HIP(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+    b->graph_steps = G;
+    b->graph_grid = step_grid(b);
+    b->graph_cur = cur_steps;
+  }
+  IDMX_HIP(hipEventRecord(b->ctx->ev0, st));
   if (pull) {
-    // to the pull form; the buffer flips, so a graph captured after this
-    // point always sees the same parity (the conversion back flips it again)
+    // to the pull form (the buffer flips; the conversion back flips it again)
     hipError_t e = restream(b, 1);
     if (e != hipSuccess) return hip_fail(e, "idmx_batch_run restream");
     b->cur ^= 1;
   }
   if (b->n_items > 0 && nsteps > 0) {
-    // graph of an even number of steps (A->B, B->A, ...): replays keep the
-    // state in pop[cur]
-    const int G = 2 * std::max(1, std::min(nsteps / 2, 50));
-    if (nsteps >= G && b->graph_steps != G) {
-      drop_graph(b);
-      hipGraph_t g;
-      IDMX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < G; k++) {
-        const int src = (b->cur + k) & 1;
-        hipError_t e = one_step(b, b->pop[src], b->pop[src ^ 1]);
-        if (e != hipSuccess) {
-          hipStreamEndCapture(st, &g);
-          return hip_fail(e, "idmx_batch_run capture");
-        }
-      }
-      IDMX_HIP(hipStreamEndCapture(st, &g));
-      IDMX_HIP(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
-      hipGraphDestroy(g);
-      b->graph_steps = G;
-      b->graph_grid = step_grid(b);
-    }
     int done = 0;
-    // the graph was captured for the parity the state had then; cur is
-    // unchanged after every replay, so it matches
-    while (b->graph && nsteps - done >= b->graph_steps) {
+    // a graph replays the buffers it was captured with; cur is unchanged
+    // after every replay
+    while (b->graph && b->graph_cur == b->cur && nsteps - done >= b->graph_steps) {
       IDMX_HIP(hipGraphLaunch(b->graph, st));
       done += b->graph_steps;
     }
@@ -300,13 +306,23 @@ int idmx_batch_run(idmx_batch_t b, int nsteps) {
       hipError_t e = one_step(b, b->pop[b->cur], b->pop[b->cur ^ 1]);
       if (e != hipSuccess) return hip_fail(e, "idmx_batch_run launch");
       b->cur ^= 1;
-      drop_graph(b);  // parity changed: the captured buffers no longer match
     }
   }
   if (pull) {
     hipError_t e = restream(b, -1);
     if (e != hipSuccess) return hip_fail(e, "idmx_batch_run restream");
     b->cur ^= 1;
+  }
+  if (b->cur != cur_in) {
+    // odd number of single steps: carry the inactive domains' state over
+    // into the buffer that is now current
+    for (int d = 0; d < b->ndom; d++) {
+      if (b->active[d]) continue;
+      const BatchDom& D = b->doms[d];
+      const size_t off = b->elem * (size_t)D.pop_off;
+      IDMX_HIP(hipMemcpyAsync((char*)b->pop[b->cur] + off, (const char*)b->pop[cur_in] + off,
+                              b->elem * 9 * (size_t)D.L0 * D.L1, hipMemcpyDeviceToDevice, st));
+    }
   }
   IDMX_HIP(hipEventRecord(b->ctx->ev1, st));
   IDMX_HIP(hipEventSynchronize(b->ctx->ev1));

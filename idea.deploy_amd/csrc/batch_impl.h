@@ -37,6 +37,7 @@ struct idmx_batch_s {
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
   int graph_grid = 0;  // step grid the graph was captured with
+  int graph_cur = 0;   // buffer holding the state when the graph starts
   double last_ms = 0;
 };
 

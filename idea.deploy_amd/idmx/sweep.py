@@ -99,11 +99,16 @@ class Batch:
         return tuple(x.value for x in v)
 
     def set_active(self, active):
-        a = np.ascontiguousarray(np.asarray(active, dtype=np.int32))
+        a = np.ascontiguousarray(np.asarray(active, dtype=np.int32).ravel())
+        if a.size != self.n:
+            raise ValueError('set_active: %d flags for %d domains' % (a.size, self.n))
         _lib.call('idmx_batch_set_active', self.handle, a.ctypes.data_as(C.POINTER(C.c_int)))
 
     def set_density(self, i, n):
         a = np.ascontiguousarray(np.asarray(n, dtype=np.float64).ravel())
+        L = self.points[i].L
+        if a.size != L * L:
+            raise ValueError('set_density: %d values for a %dx%d domain' % (a.size, L, L))
         _lib.call('idmx_batch_set_density', self.handle, int(i), a.ctypes.data)
 
     def get_density(self, i):
@@ -240,6 +245,10 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     warm = np.array([start is not None and start[i] is not None for i in range(n)])
     init = [cold_start(p) if not warm[i] else np.asarray(start[i], dtype=np.float64).ravel()
             for i, p in enumerate(points)]
+    for i, p in enumerate(points):
+        if init[i].size != p.L * p.L:
+            raise ValueError('run_sweep: start %d has %d values for a %dx%d domain'
+                             % (i, init[i].size, p.L, p.L))
     fb_all = np.zeros(n, dtype=bool)
     for i, p in enumerate(points):
         if warm[i] and not np.isfinite(init[i]).all():   # unusable warm start

@@ -226,6 +226,126 @@ def test_mrt_basis_is_w_orthogonal_hermite(name):
     assert [M[tuples.index((0,)), q] for q in range(Q)] == [v[0] for v in hm.XI[name]]
 
 
+# Closed-form tensor Hermite polynomials c_s^n H^(n)(xi / c_s), c2 = 1/3, as
+# products of the one-dimensional ones (index counts (kx, ky, kz)):
+# h0 = 1, h1 = x, h2 = x^2 - c2, h3 = x^3 - 3 c2 x, h4 = x^4 - 6 c2 x^2 + 3 c2^2.
+_C2 = Fr(1, 3)
+_H1D = (lambda x: Fr(1), lambda x: Fr(x), lambda x: x * x - _C2,
+        lambda x: x ** 3 - 3 * _C2 * x, lambda x: x ** 4 - 6 * _C2 * x * x + 3 * _C2 * _C2)
+
+
+def _hermite_closed(tup, v):
+    out = Fr(1)
+    for a in range(len(v)):
+        out *= _H1D[tup.count(a)](Fr(v[a]))
+    return out
+
+
+# The rows of the non-orthogonal invertible basis M' in the reference's order
+# (GetInvertibleHermiteSet, idpy/IdpyStencils/IdpyStencils.py:905-971:
+# rank by rank, Taylor-tuple order within a rank, rows that vanish on the
+# lattice skipped, then the rref pivot rows), written out by hand.
+_MP_ROWS = {
+    'D2Q9': [
+        ((), lambda x, y: 1),
+        ((0,), lambda x, y: x),
+        ((1,), lambda x, y: y),
+        ((0, 0), lambda x, y: x * x - _C2),
+        ((0, 1), lambda x, y: x * y),
+        ((1, 1), lambda x, y: y * y - _C2),
+        ((0, 0, 1), lambda x, y: x * x * y - _C2 * y),
+        ((0, 1, 1), lambda x, y: x * y * y - _C2 * x),
+        ((0, 0, 1, 1), lambda x, y: x * x * y * y - _C2 * (x * x + y * y) + _C2 * _C2),
+    ],
+    'D3Q19': [
+        ((), lambda x, y, z: 1),
+        ((0,), lambda x, y, z: x),
+        ((1,), lambda x, y, z: y),
+        ((2,), lambda x, y, z: z),
+        ((0, 0), lambda x, y, z: x * x - _C2),
+        ((0, 1), lambda x, y, z: x * y),
+        ((0, 2), lambda x, y, z: x * z),
+        ((1, 1), lambda x, y, z: y * y - _C2),
+        ((1, 2), lambda x, y, z: y * z),
+        ((2, 2), lambda x, y, z: z * z - _C2),
+        ((0, 0, 1), lambda x, y, z: x * x * y - _C2 * y),
+        ((0, 0, 2), lambda x, y, z: x * x * z - _C2 * z),
+        ((0, 1, 1), lambda x, y, z: x * y * y - _C2 * x),
+        ((0, 2, 2), lambda x, y, z: x * z * z - _C2 * x),
+        ((1, 1, 2), lambda x, y, z: y * y * z - _C2 * z),
+        ((1, 2, 2), lambda x, y, z: y * z * z - _C2 * y),
+        ((0, 0, 1, 1), lambda x, y, z: x * x * y * y - _C2 * (x * x + y * y) + _C2 * _C2),
+        ((0, 0, 2, 2), lambda x, y, z: x * x * z * z - _C2 * (x * x + z * z) + _C2 * _C2),
+        ((1, 1, 2, 2), lambda x, y, z: y * y * z * z - _C2 * (y * y + z * z) + _C2 * _C2),
+    ],
+}
+
+
+def _frac_rank(rows):
+    import sympy as sp
+    return sp.Matrix([[sp.Rational(x.numerator, x.denominator) for x in r] for r in rows]).rank()
+
+
+@pytest.mark.parametrize('name', ['D2Q9', 'D3Q19'])
+def test_mrt_basis_rows_closed_form(name):
+    """Every row of M' (oracle and package) equals the hand-written Hermite
+    polynomial on the lattice, in the hand-written order; every candidate of
+    rank <= 4 left out is zero on the lattice or depends on the rows kept
+    before it (the greedy rref pivot rule); the W-orthogonal basis is
+    Mw = L M' with L unit lower-triangular and Mw W Mw^T diagonal, which
+    fixes the Gram-Schmidt result uniquely."""
+    from itertools import combinations_with_replacement
+    import sympy as sp
+    from idmx.LBM import hermite
+    from oracle import hermite_mrt as hm
+    xis = hm.XI[name]
+    D = len(xis[0])
+    want = _MP_ROWS[name]
+    Q = len(xis)
+    assert len(want) == Q
+    rows_cf = [[Fr(fn(*[Fr(c) for c in v])) for v in xis] for _, fn in want]
+    # the hand-written polynomials are the product-form Hermite tensors
+    for (tup, _), r in zip(want, rows_cf):
+        assert r == [_hermite_closed(tup, v) for v in xis]
+    # oracle basis
+    Mp, tuples = hm.invertible_set(name)
+    assert [tuple(t) for t in tuples] == [t for t, _ in want]
+    assert all(Mp[i, q] == sp.Rational(rows_cf[i][q].numerator, rows_cf[i][q].denominator)
+               for i in range(Q) for q in range(Q))
+    # package basis (its own velocity order)
+    pk_rows, pk_tuples = hermite.invertible_set(name)
+    from idmx.LBM.stencils import XIStencils
+    pk_xis = XIStencils[name]['XIs']
+    assert list(pk_tuples) == [t for t, _ in want]
+    for (tup, fn), r in zip(want, pk_rows):
+        assert list(r) == [Fr(fn(*[Fr(c) for c in v])) for v in pk_xis]
+    # selection: a candidate not kept is zero on the lattice or dependent on
+    # the rows kept before it in Taylor-tuple order
+    kept = [t for t, _ in want]
+    for order in range(1, 5):
+        for tup in combinations_with_replacement(range(D), order):
+            if tup in kept:
+                continue
+            r = [_hermite_closed(tup, v) for v in xis]
+            if all(x == 0 for x in r):
+                continue
+            before = [rows_cf[i] for i, t in enumerate(kept)
+                      if (len(t), t) < (len(tup), tup)]
+            assert _frac_rank(before + [r]) == _frac_rank(before), tup
+    # Gram-Schmidt: Mw = L M', L unit lower-triangular, Mw W Mw^T diagonal
+    rat = lambda rows: sp.Matrix([[sp.Rational(x.numerator, x.denominator) for x in r]
+                                  for r in rows])
+    pk_W = sp.diag(*[sp.Rational(w).limit_denominator(1000) for w in XIStencils[name]['Ws']])
+    for Mw, Mpb, W in ((hm.worth_set(name), Mp, sp.diag(*hm.W[name])),
+                       (rat(hermite.worth_set(name)[0]), rat(pk_rows), pk_W)):
+        L = Mw * Mpb.inv()
+        assert all(L[i, i] == 1 for i in range(Q))
+        assert all(L[i, j] == 0 for i in range(Q) for j in range(i + 1, Q))
+        Gm = Mw * W * Mw.T
+        assert all(Gm[i, j] == 0 for i in range(Q) for j in range(Q) if i != j)
+        assert all(Gm[i, i] > 0 for i in range(Q))
+
+
 def test_mrt_with_uniform_rates_is_bgk():
     """S = omega I: A = (1-omega) I, B = omega I, Gg = (1-omega/2) I, so the
     MRT step equals the BGK step (Collisions.py:96-160 vs 163-185)."""
