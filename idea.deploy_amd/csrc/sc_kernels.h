@@ -97,6 +97,30 @@ __device__ inline void basis_coords(T dn, T n, T ux, T uy, T uz, T Fx, T Fy, T F
   }
 }
 
+// Sum of terms with compile-time coefficients, started from its first
+// non-zero term (0 + a is not folded to a: signed zeros), coefficients +-1
+// applied as additions and subtractions. With the loops unrolled every
+// branch on `any` and on the coefficient is resolved at compile time.
+template <class T>
+struct Acc {
+  T v = T(0);
+  bool any = false;
+  __device__ __forceinline__ void add(T t) {
+    v = any ? v + t : t;
+    any = true;
+  }
+  __device__ __forceinline__ void sub(T t) {
+    v = any ? v - t : -t;
+    any = true;
+  }
+  __device__ __forceinline__ void addc(double c, T t) {
+    if (c == 1.0) add(t);
+    else if (c == -1.0) sub(t);
+    else if (c != 0.0) add(T(c) * t);
+  }
+  __device__ __forceinline__ T get() const { return any ? v : T(0); }
+};
+
 // True when moment k of some forcing basis vector is non-zero (Mp v_b).
 template <class S>
 __device__ constexpr bool mpv_row_nz(int k) {
@@ -236,41 +260,96 @@ __device__ inline void collide(const T (&f)[S::Q], T dn, T gx, T gy, T gz,
     // COLL 4: BGK with Guo forcing, then the isotropy filter (D2Q9, FP64)
     if constexpr (COLL == 4) iso_filter<S, T>(fo);
   } else if (COLL == 2) {
-    // Diagonal S in moment space with the bases as compile-time tables:
-    // f* = f - Mw^-1 [s o (Mw f - Mw f_eq)] + Mp^-1 [(1 - s/2) o (Mp F)],
-    // Mw f_eq = sum_k c_k (Mw v_k), Mp F = sum_k d_k (Mp v_k) (F the raw Guo
-    // term); P.mrtA holds the Q rates s.
-    constexpr int NB = NBasis<S::D>::value;
-    T c[NB], d[NB];
-    basis_coords<S, T, SHIFT>(dn, n, ux, uy, uz, Fx, Fy, Fz, uu, Fu, c, d);
+    // Diagonal S in moment space:
+    //   f* = f - Mw^-1 [s o (Mw f - Mw f_eq)] + Mp^-1 [(1 - s/2) o (Mp F)]
+    // (F the raw Guo term; P.mrtA holds the Q rates s), evaluated through
+    // the lattice monomials: Mw = Tm Mm with Mm[k][q] = xi_q^e_k in
+    // {-1, 0, 1}, Mw = Lt Mp, Mw^-1 = W Mw^T diag(Dinv). Then
+    //   f* = f + W Mm^T Tm^T b,  b = Dinv o (Lt mg - md),
+    // md = s o (Mw f - Mw f_eq), mg = (1 - s/2) o Mp F. The monomial sums
+    // run over opposite pairs (even monomials take f_q + f_-q, odd ones
+    // f_q - f_-q), so each moment is a few additions. Row 0 (the mass) is
+    // not relaxed: Mw f and Mw f_eq agree there, as do Mp F and 0.
+    constexpr int NP = S::NPAIR;
     const T* s = P.mrtA;
-    T md[Q], mg[Q];
+    T ps[NP], pd[NP];
 #pragma unroll
-    for (int k = 0; k < Q; k++) {
-      T m = 0;
-#pragma unroll
-      for (int q = 0; q < Q; q++)
-        if (S::Mw[k][q] != 0.0) m += T(S::Mw[k][q]) * f[q];
-#pragma unroll
-      for (int b = 0; b < NB; b++)
-        if (S::MwV[b][k] != 0.0) m -= T(S::MwV[b][k]) * c[b];
-      md[k] = s[k] * m;
-      T g = 0;
-#pragma unroll
-      for (int b = 0; b < NB; b++)
-        if (S::MpV[b][k] != 0.0) g += T(S::MpV[b][k]) * d[b];
-      mg[k] = (T(1) - T(0.5) * s[k]) * g;
+    for (int p = 0; p < NP; p++) {
+      ps[p] = f[S::PairQ[p]] + f[S::opp[S::PairQ[p]]];
+      pd[p] = f[S::PairQ[p]] - f[S::opp[S::PairQ[p]]];
     }
+    // monomial moments R_k = sum_q xi_q^e_k f_q; R_0 = sum_q f_q (shifted
+    // populations give the same W-orthogonal moments for k >= 1)
+    T R[Q];
+    R[0] = dn;
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-      T a = f[q];
+    for (int k = 1; k < Q; k++) {
+      Acc<T> a;
 #pragma unroll
-      for (int k = 0; k < Q; k++)
-        if (S::Mwi[q][k] != 0.0) a -= T(S::Mwi[q][k]) * md[k];
+      for (int p = 0; p < NP; p++)
+        a.addc(S::Mm[k][S::PairQ[p]], S::MonoOdd[k] ? pd[p] : ps[p]);
+      R[k] = a.get();
+    }
+    // non-equilibrium W-orthogonal moments; Mw f_eq is n, n u_a, n u_a u_b
+    // on the rows up to second order and 0 above
+    const T nu[3] = {n * ux, n * uy, n * uz};
+    const T uv[3] = {ux, uy, uz};
+    T md[Q];
 #pragma unroll
-      for (int k = 0; k < Q; k++)
-        if (S::Mpi[q][k] != 0.0 && mpv_row_nz<S>(k)) a += T(S::Mpi[q][k]) * mg[k];
-      fo[q] = a;
+    for (int k = 1; k < Q; k++) {
+      Acc<T> a;
+#pragma unroll
+      for (int j = 0; j < Q; j++) a.addc(S::Tm[k][j], R[j]);
+      T m = a.get();
+      if (S::HTord[k] == 1) m = m - nu[S::HTa[k]];
+      if (S::HTord[k] == 2) m = m - nu[S::HTa[k]] * uv[S::HTb[k]];
+      md[k] = s[k] * m;
+    }
+    // Mp F from the coordinates d of the raw Guo term on w o {1, xi, xi xi}
+    constexpr int NB = NBasis<S::D>::value;
+    T cb[NB], d[NB];
+    basis_coords<S, T, SHIFT>(dn, n, ux, uy, uz, Fx, Fy, Fz, uu, Fu, cb, d);
+    T mg[Q];
+#pragma unroll
+    for (int k = 1; k < Q; k++) {
+      if (!mpv_row_nz<S>(k)) continue;
+      Acc<T> a;
+#pragma unroll
+      for (int b = 0; b < NB; b++) a.addc(S::MpV[b][k], d[b]);
+      mg[k] = (T(1) - T(0.5) * s[k]) * a.get();
+    }
+    T bk[Q];
+#pragma unroll
+    for (int k = 1; k < Q; k++) {
+      Acc<T> l;
+      if (mpv_row_nz<S>(k)) l.add(mg[k]);
+#pragma unroll
+      for (int j = 1; j < k; j++)
+        if (mpv_row_nz<S>(j)) l.addc(S::Lt[k][j], mg[j]);
+      bk[k] = T(S::Dinv[k]) * (l.any ? l.get() - md[k] : -md[k]);
+    }
+    // c = Tm^T b, then f*_q = f_q + w_q sum_k Mm[k][q] c_k pair by pair
+    T cm[Q];
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+      Acc<T> a;
+#pragma unroll
+      for (int k = 1; k < Q; k++) a.addc(S::Tm[k][j], bk[k]);
+      cm[j] = a.get();
+    }
+    fo[0] = f[0] + T(S::w[0]) * cm[0];
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      const int q = S::PairQ[p], o = S::opp[S::PairQ[p]];
+      Acc<T> E, O;
+#pragma unroll
+      for (int k = 0; k < Q; k++) {
+        if (S::MonoOdd[k]) O.addc(S::Mm[k][q], cm[k]);
+        else E.addc(S::Mm[k][q], cm[k]);
+      }
+      const T wE = T(S::w[q]) * E.get(), wO = T(S::w[q]) * O.get();
+      fo[q] = f[q] + wE + wO;
+      fo[o] = f[o] + wE - wO;
     }
   } else {
     // f* = A f + sum_k c_k beta_k + sum_k d_k gamma_k, with c_k / d_k the
@@ -327,6 +406,15 @@ __device__ __forceinline__ void pop_st_nt(T* __restrict__ p, int q, int64_t V, i
   } else {
     __builtin_nontemporal_store(v, p + (size_t)q * V + g);
   }
+}
+
+// Non-temporal store at a wave-uniform 64-bit base (SGPRs) plus a 32-bit
+// per-lane byte offset: the compiler emits global_store v_off, v_data,
+// s[base:base+1], so a site's Q stores share one offset register.
+template <class T>
+__device__ __forceinline__ void st_nt_sbase(uint64_t base, uint32_t off, T v) {
+  IDMX_GLOBAL char* b = (IDMX_GLOBAL char*)sgpr_u64(base);
+  __builtin_nontemporal_store(v, (IDMX_GLOBAL T*)(b + off));
 }
 
 // Push-stream one site's f* to its neighbours (periodic).
@@ -746,12 +834,12 @@ k_fused_o3(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<T
 //   barrier (the staging buffer is free again);
 //   issue the copy of plane j+1; collide plane j-R and push its f*.
 //
-// Waits: vector memory operations of a wave complete in issue order, so at
-// the top of an iteration vmcnt(Q) (at most the Q push stores of the last
-// collision still outstanding) means the copy, issued before them, has
-// landed. A wave that issued no stores since the copy (no collision yet, or
-// no valid site in the wave) waits for vmcnt(0). The barriers wait for LDS
-// only (lgkmcnt), not for the outstanding stores.
+// Waits: at the top of an iteration every wave waits for all of its vector
+// memory operations (vmcnt(0)): the copy and the push stores issued behind
+// it. This makes no assumption about the order in which loads and stores
+// complete; waiting only for the copy (vmcnt(Q), counting on in-order
+// completion) measured no faster, since the stores were issued a collision
+// earlier. The barriers wait for LDS only (lgkmcnt).
 //
 // Staging layout: population q of the haloed plane occupies a block of NK
 // chunks of 64 dwords (one DMA instruction each); dword d of the block is
@@ -850,8 +938,6 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
   const int ix = tid % TX, iy = tid / TX;
   const int x = x0 + ix, y = D == 3 ? y0 + iy : 0;
   const bool own_valid = x < geo.L0 && (D == 2 || y < geo.L1);
-  // wave-uniform: does this wave store anything when it collides a plane
-  const bool wave_stores = __builtin_amdgcn_ballot_w64(own_valid) != 0;
 
   // Per-lane source byte offsets (within a plane of one population) of the
   // chunks this wave fetches, and whether the lane's dword exists.
@@ -898,15 +984,49 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
   T dnh[HR];
   const int own_site = (D == 3 ? (iy + R) * PX : 0) + ix + R;
 
+  // Push targets: byte offsets within a marching plane of the site and its
+  // in-plane neighbours (periodic), computed once; the target plane of
+  // population q is wave-uniform and goes into the scalar base.
+  const int xm = wrap(x - 1, geo.L0), xp = wrap(x + 1, geo.L0);
+  const int ym = D == 3 ? wrap(y - 1, geo.L1) : 0, yp = D == 3 ? wrap(y + 1, geo.L1) : 0;
+  uint32_t toff[3][3];  // [dy + 1][dx + 1]
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    const int yy = b == 0 ? ym : b == 1 ? y : yp;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const int xx = a == 0 ? xm : a == 1 ? x : xp;
+      toff[b][a] = ((uint32_t)xx + (uint32_t)geo.L0 * (uint32_t)(D == 3 ? yy : 0)) *
+                   (uint32_t)sizeof(T);
+    }
+  }
+  const uint64_t qbytes = (uint64_t)geo.V * sizeof(T);
+  // f* of the last collided plane, stored after the next copy is issued so
+  // that the stores and the copy's loads are in flight together
+  T fpend[Q];
+  int pend_m = -1;
+  auto flush = [&]() {
+    if (pend_m >= 0 && own_valid) {
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const int cm = D == 3 ? S::cz[q] : S::cy[q];
+        const int mt = cm ? wrap(pend_m + cm, Lm) : pend_m;
+        const int dy = D == 3 ? S::cy[q] : 0;
+        uint64_t base = (uint64_t)(pop_swap) + (uint64_t)q * qbytes +
+                        (uint64_t)mt * (uint64_t)PS * sizeof(T);
+        st_nt_sbase(base, toff[dy + 1][S::cx[q] + 1], fpend[q]);
+      }
+    }
+    pend_m = -1;
+  };
+
   issue_copy(0);
-  bool stored = false;  // this wave issued stores after the last copy
   for (int j = 0; j < J; j++) {
     // raised wave priority from here until the next copy is issued: this
     // stretch is on the critical path of the block, while the other block
     // on the CU may be colliding (measured 1-5 % faster, steadier)
     asm volatile("s_setprio 2" ::: "memory");
-    if (stored) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(Q) : "memory");
-    else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     lds_barrier();
     const int slot = j % NS;
     {
@@ -947,7 +1067,8 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
     lds_barrier();
     if (j + 1 < J) issue_copy(j + 1);
     asm volatile("s_setprio 0" ::: "memory");
-    stored = false;
+    // the previous plane's stores, issued behind the copy
+    flush();
     if (j >= 2 * R) {
       if (own_valid) {
         const int mc = m0 + j - 2 * R;
@@ -965,12 +1086,10 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
           if (ES::ez[e] != 0) gz += T(ES::ez[e]) * w * v;
         }
         const T lpsi = ring[(j - R) % NS][cy][cx];
-        T fo[Q], u[3];
-        collide<S, T, SHIFT, COLL>(fh[0], dnh[0], gx, gy, gz, lpsi, PL, fo, u);
-        const int xc = x, yc = D == 3 ? y : mc, zc = D == 3 ? mc : 0;
-        push_store<S, T>(pop_swap, geo, xc, yc, zc, fo);
+        T u[3];
+        collide<S, T, SHIFT, COLL>(fh[0], dnh[0], gx, gy, gz, lpsi, PL, fpend, u);
       }
-      stored = wave_stores && j + 1 < J;
+      pend_m = m0 + j - 2 * R;
     }
 #pragma unroll
     for (int k = 0; k < HR - 1; k++) {
@@ -979,6 +1098,7 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
       for (int q = 0; q < Q; q++) fh[k][q] = fh[k + 1][q];
     }
   }
+  flush();
 }
 
 // --------------------------------------------------------------------------

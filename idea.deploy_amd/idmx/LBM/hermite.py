@@ -19,6 +19,7 @@ non-orthogonalised basis) is reduced to the three tables the kernel takes:
 A = M^-1 (1-S) M, beta_k = M^-1 S M v_k, gamma_k = M'^-1 (1-S/2) M' v_k with
 v_k = w * {1, xi_a, xi_a xi_b (a <= b)}.
 """
+import math
 from fractions import Fraction as Fr
 from functools import lru_cache
 from itertools import combinations_with_replacement
@@ -202,3 +203,77 @@ def mrt_kernel_tables(name, S):
     if np.count_nonzero(S - np.diag(np.diag(S))) == 0:
         return {'diag': np.ascontiguousarray(np.diag(S))}
     return mrt_tables(name, S)
+
+
+# Lattice monomials xi_x^a xi_y^b (xi_z^c), exponents <= 2, whose values on
+# the velocity set form an invertible Q x Q matrix with entries in {-1, 0, 1}.
+MONOMIALS = {
+    'D2Q9': ((0, 0), (1, 0), (0, 1), (2, 0), (0, 2), (1, 1), (2, 1), (1, 2), (2, 2)),
+    'D3Q19': ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2),
+              (1, 1, 0), (1, 0, 1), (0, 1, 1), (2, 1, 0), (2, 0, 1), (1, 2, 0), (1, 0, 2),
+              (0, 2, 1), (0, 1, 2), (2, 2, 0), (2, 0, 2), (0, 2, 2)),
+}
+
+
+@lru_cache(maxsize=None)
+def monomial_tables(name):
+    """Exact tables of the moment-space MRT collision.
+
+    With Mm[k][q] = xi_q^e_k (the lattice monomials), the W-orthogonal basis
+    is Mw = Tm Mm, the non-orthogonal one Mp = Lt^-1 Mw (Lt unit lower-
+    triangular), and Mw W Mw^T = diag(1 / Dinv), so that
+        f* = f - Mw^-1 s(Mw f - Mw f_eq) + Mp^-1 (1 - s/2) Mp F
+           = f + W Mm^T Tm^T Dinv (Lt mg - md)
+    with md = s (Mw f - Mw f_eq) and mg = (1 - s/2) Mp F. Returns the
+    integer Mm, the parity of each monomial, Tm, Lt and Dinv as Fractions,
+    the Hermite tuple (order, a, b) of every row up to second order, and
+    the first velocity of every opposite pair."""
+    st = XIStencils[name]
+    xis = st['XIs']
+    Q, D = len(xis), len(xis[0])
+    mons = MONOMIALS[name]
+    Mm = [[Fr(math.prod(v[a] ** e[a] for a in range(D))) for v in xis] for e in mons]
+    Mw, _ = worth_set(name)
+    Mp, tuples = invertible_set(name)
+    Mmi = _inv(Mm)
+    mul = lambda A, B: [[sum(A[i][k] * B[k][j] for k in range(len(B)))
+                         for j in range(len(B[0]))] for i in range(len(A))]
+    Tm = mul([list(r) for r in Mw], Mmi)
+    Lt = mul([list(r) for r in Mw], _inv([list(r) for r in Mp]))
+    for i in range(Q):
+        if Lt[i][i] != 1 or any(Lt[i][j] != 0 for j in range(i + 1, Q)):
+            raise RuntimeError('Mw is not a unit lower-triangular map of Mp')
+    _, ws, _ = _lattice(name)
+    Dg = [sum(Mw[k][q] * Mw[k][q] * ws[q] for q in range(Q)) for k in range(Q)]
+    for k in range(Q):
+        for j in range(k):
+            if sum(Mw[k][q] * Mw[j][q] * ws[q] for q in range(Q)) != 0:
+                raise RuntimeError('Mw is not W-orthogonal')
+    xs = [tuple(v) for v in xis]
+    opp = [xs.index(tuple(-c for c in v)) for v in xs]
+    pairs = [q for q in range(Q) if q < opp[q]]
+    ht_ord, ht_a, ht_b = [], [], []
+    for t in tuples:
+        ht_ord.append(len(t) if len(t) <= 2 else 3)
+        ht_a.append(t[0] if 1 <= len(t) <= 2 else -1)
+        ht_b.append(t[1] if len(t) == 2 else -1)
+    return {'Mm': [[int(x) for x in r] for r in Mm],
+            'odd': [sum(e) % 2 for e in mons], 'Tm': Tm, 'Lt': Lt,
+            'Dinv': [1 / d for d in Dg], 'ht_ord': ht_ord, 'ht_a': ht_a, 'ht_b': ht_b,
+            'pairs': pairs}
+
+
+@lru_cache(maxsize=None)
+def basis_moments(name):
+    """(MwV, MpV): the W-orthogonal and the non-orthogonal moments of the
+    basis vectors v_b = w * {1, xi_a, xi_a xi_b (a <= b)} (in the order of
+    basis_vectors), as exact Fractions, NB x Q."""
+    xis, ws, D = _lattice(name)
+    Q = len(xis)
+    mons = [()] + [(a,) for a in range(D)] + [(a, a) for a in range(D)] + \
+        [(a, b) for a in range(D) for b in range(a + 1, D)]
+    V = [[ws[q] * math.prod(Fr(xis[q][a]) for a in m) for q in range(Q)] for m in mons]
+    Mw, _ = worth_set(name)
+    Mp, _ = invertible_set(name)
+    mom = lambda M: [[sum(M[k][q] * v[q] for q in range(Q)) for k in range(Q)] for v in V]
+    return mom(Mw), mom(Mp)

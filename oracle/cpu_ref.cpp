@@ -341,6 +341,45 @@ struct MRTMats {
   const double* Gg;  // M'^-1 (1-S/2) M'
 };
 
+// The same MRT collision evaluated in moment space (diagonal S, rates s):
+//   f* = f - Mw^-1 [s o (Mw f - Mw f_eq)] + Mp^-1 [(1 - s/2) o (Mp F_raw)]
+// written through the lattice monomials (Mw = Tm Mm, Mm[k][q] = xi_q^e_k),
+// Mw = Lt Mp and Mw^-1 = W Mw^T diag(Dinv):
+//   f* = f + W Mm^T Tm^T b,  b = Dinv o (Lt mg - md),
+//   md = s o (Mw f - Mw f_eq),  mg = (1 - s/2) o (Mp F_raw),
+// with Mw f_eq = (n, n u_a, n u_a u_b, 0, ...) on the Hermite rows and
+// Mp F_raw from the coordinates of F_raw on w o {1, xi_a, xi_a xi_b}. The
+// tables come from oracle/hermite_mrt.py (its own sympy bases). Every sum
+// starts from its first non-zero term and runs in index order: monomial
+// sums over opposite pairs (f_q + f_-q for even monomials, f_q - f_-q for
+// odd ones), so that the HIP kernel's moment-space collision, which
+// evaluates the same sums in the same order, can be followed closely over
+// long runs; the dense path above is the independent check of the algebra.
+struct MomTabs {
+  int NP;
+  const int* pairs;   // first velocity of each opposite pair
+  const int* mm;      // Q x Q monomial values in {-1, 0, 1}
+  const int* odd;     // Q: parity of each monomial
+  const double* tm;   // Q x Q
+  const double* lt;   // Q x Q, unit lower-triangular
+  const double* dinv; // Q
+  const int* ht;      // 3 x Q: Hermite order (0..2, 3 = higher), a, b
+  const double* mpv;  // NB x Q: Mp moments of the basis vectors
+  const double* s;    // Q relaxation rates
+};
+
+struct Sum {
+  double v = 0.;
+  bool any = false;
+  void add(double c, double t) {
+    if (c == 0.) return;
+    const double x = c == 1. ? t : c == -1. ? -t : c * t;
+    v = any ? (c == -1. ? v - t : v + x) : x;
+    any = true;
+  }
+  double get() const { return any ? v : 0.; }
+};
+
 // Index of the opposite velocity (IdpyStencil.opposite).
 template <class S>
 int opposite(int q) {
@@ -391,11 +430,121 @@ inline void iso_filter_site(const IsoMats& t, double* f) {
 // 1099-1223): K_ForceCollideSCMPMeta collides in place, K_IsotropyFilter
 // filters, K_StreamPeriodicMeta push-streams -- the same as filtering f*
 // between the collision and the push of the fused step.
+// Sparse form of the moment-space tables: for every sum, its non-zero
+// terms in increasing index order (the order the sums are evaluated in).
+struct Term {
+  int i;
+  double c;
+};
+struct MomSparse {
+  std::vector<Term> R[19], Tm[19], Mp[19], Lt[19], TmT[19], E[9], O[9];
+  bool col[19];
+  int opp[19];
+};
+
+template <class S>
+MomSparse mom_sparse(const MomTabs& t) {
+  constexpr int Q = S::Q, NB = S::D == 2 ? 6 : 10;
+  MomSparse m;
+  for (int q = 0; q < Q; q++) m.opp[q] = opposite<S>(q);
+  for (int k = 0; k < Q; k++) {
+    m.col[k] = false;
+    for (int b = 0; b < NB; b++)
+      if (t.mpv[b * Q + k] != 0.) {
+        m.col[k] = true;
+        m.Mp[k].push_back({b, t.mpv[b * Q + k]});
+      }
+    for (int p = 0; p < t.NP; p++)
+      if (t.mm[k * Q + t.pairs[p]]) m.R[k].push_back({p, double(t.mm[k * Q + t.pairs[p]])});
+    for (int j2 = 0; j2 < Q; j2++)
+      if (t.tm[k * Q + j2] != 0.) m.Tm[k].push_back({j2, t.tm[k * Q + j2]});
+  }
+  for (int k = 0; k < Q; k++)
+    for (int j2 = 1; j2 < k; j2++)
+      if (m.col[j2] && t.lt[k * Q + j2] != 0.) m.Lt[k].push_back({j2, t.lt[k * Q + j2]});
+  for (int j2 = 0; j2 < Q; j2++)
+    for (int k = 1; k < Q; k++)
+      if (t.tm[k * Q + j2] != 0.) m.TmT[j2].push_back({k, t.tm[k * Q + j2]});
+  for (int p = 0; p < t.NP; p++)
+    for (int k = 0; k < Q; k++) {
+      const int c = t.mm[k * Q + t.pairs[p]];
+      if (c) (t.odd[k] ? m.O[p] : m.E[p]).push_back({k, double(c)});
+    }
+  return m;
+}
+
+inline double sum_terms(const std::vector<Term>& ts, const double* x) {
+  Sum a;
+  for (const Term& e : ts) a.add(e.c, x[e.i]);
+  return a.get();
+}
+
+template <class S>
+void mrt_moment(const MomTabs& t, const MomSparse& sp, const double* f, double n,
+                const double* u, const double* F, double* fo) {
+  constexpr int D = S::D, Q = S::Q;
+  double ps[9], pd[9];
+  for (int p = 0; p < t.NP; p++) {
+    const int q = t.pairs[p], o = sp.opp[q];
+    ps[p] = f[q] + f[o];
+    pd[p] = f[q] - f[o];
+  }
+  double R[Q];
+  double dn = 0.;
+  for (int q = 0; q < Q; q++) dn += f[q];
+  R[0] = dn;
+  for (int k = 1; k < Q; k++) R[k] = sum_terms(sp.R[k], t.odd[k] ? pd : ps);
+  const double nu[3] = {n * u[0], n * u[1], D == 3 ? n * u[2] : 0.};
+  double md[Q];
+  for (int k = 1; k < Q; k++) {
+    double m = sum_terms(sp.Tm[k], R);
+    const int ord = t.ht[k], ha = t.ht[Q + k], hb = t.ht[2 * Q + k];
+    if (ord == 1) m = m - nu[ha];
+    if (ord == 2) m = m - nu[ha] * u[hb];
+    md[k] = t.s[k] * m;
+  }
+  // coordinates of the raw Guo term on w o {1, xi_a, xi_a xi_b}
+  const double ux = u[0], uy = u[1], uz = D == 3 ? u[2] : 0.;
+  const double Fx = F[0], Fy = F[1], Fz = D == 3 ? F[2] : 0.;
+  double d[10];
+  d[0] = -3. * (Fx * ux + Fy * uy + Fz * uz);
+  if (D == 2) {
+    d[1] = 3. * Fx; d[2] = 3. * Fy;
+    d[3] = 9. * Fx * ux; d[4] = 9. * Fy * uy; d[5] = 9. * (Fx * uy + Fy * ux);
+  } else {
+    d[1] = 3. * Fx; d[2] = 3. * Fy; d[3] = 3. * Fz;
+    d[4] = 9. * Fx * ux; d[5] = 9. * Fy * uy; d[6] = 9. * Fz * uz;
+    d[7] = 9. * (Fx * uy + Fy * ux);
+    d[8] = 9. * (Fx * uz + Fz * ux);
+    d[9] = 9. * (Fy * uz + Fz * uy);
+  }
+  double mg[Q];
+  for (int k = 1; k < Q; k++)
+    if (sp.col[k]) mg[k] = (1. - 0.5 * t.s[k]) * sum_terms(sp.Mp[k], d);
+  double bk[Q];
+  for (int k = 1; k < Q; k++) {
+    Sum l;
+    if (sp.col[k]) l.add(1., mg[k]);
+    for (const Term& e : sp.Lt[k]) l.add(e.c, mg[e.i]);
+    bk[k] = t.dinv[k] * (l.any ? l.get() - md[k] : -md[k]);
+  }
+  double cm[Q];
+  for (int j = 0; j < Q; j++) cm[j] = sum_terms(sp.TmT[j], bk);
+  fo[0] = f[0] + S::w[0] * cm[0];
+  for (int p = 0; p < t.NP; p++) {
+    const int q = t.pairs[p], o = sp.opp[q];
+    const double wE = S::w[q] * sum_terms(sp.E[p], cm), wO = S::w[q] * sum_terms(sp.O[p], cm);
+    fo[q] = f[q] + wE + wO;
+    fo[o] = f[o] + wE - wO;
+  }
+}
+
 template <class S>
 void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
                double omega, int psi_kind, const MRTMats* mrt, double*& pop,
                double*& pop_swap, double* n, double* psi, const uint8_t* walls,
-               const IsoMats* iso = nullptr) {
+               const IsoMats* iso = nullptr, const MomTabs* mom = nullptr,
+               const MomSparse* msp = nullptr) {
   constexpr int D = S::D, Q = S::Q;
   const int64_t V = geo.V;
   // K_ComputeDensityPsiMeta (MultiPhaseKernelsMeta.py:107-167); with walls
@@ -430,7 +579,9 @@ void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
     }
     double feq[Q], fg[Q], fout[Q];
     for (int q = 0; q < Q; q++) eq_guo<S>(q, ln, lu, F, u_dot_u, F_dot_u, &feq[q], &fg[q]);
-    if (!mrt) {
+    if (mom) {
+      mrt_moment<S>(*mom, *msp, lpop, ln, lu, F, fout);
+    } else if (!mrt) {
       // SRT: (1-w) f + w f_eq + (1-w/2) F_guo (Collisions.py:163-185)
       for (int q = 0; q < Q; q++)
         fout[q] = lpop[q] * (1. - omega) + feq[q] * omega + (1. - 0.5 * omega) * fg[q];
@@ -606,6 +757,34 @@ int cpuref_meta_run(int xi_id, const int* L, int QE, const int* E,
   for (int s = 0; s < nsteps; s++) {
     if (xi_id == 0) meta_step<SD2Q9>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi, walls);
     else meta_step<SD3Q19>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi, walls);
+  }
+  if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
+  return 0;
+}
+
+// nsteps of MainLoopMRT with a diagonal relaxation matrix, the collision
+// evaluated in moment space (mrt_moment above). Table layout as MomTabs.
+int cpuref_meta_run_mom(int xi_id, const int* L, int QE, const int* E, const double* EW,
+                        double G, int psi_kind, int aos, int NP, const int* pairs,
+                        const int* mm, const int* odd, const double* tm, const double* lt,
+                        const double* dinv, const int* ht, const double* mpv, const double* s,
+                        double* pop, double* pop_swap, double* n, double* psi, int nsteps) {
+  Geo geo;
+  if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
+  const int Q = xi_id == 0 ? 9 : 19;
+  if (NP != (Q - 1) / 2) return -1;
+  EStencil es{QE, E, EW};
+  Ord ord{aos != 0, geo.V, Q};
+  MomTabs t{NP, pairs, mm, odd, tm, lt, dinv, ht, mpv, s};
+  const MomSparse sp = xi_id == 0 ? mom_sparse<SD2Q9>(t) : mom_sparse<SD3Q19>(t);
+  double *a = pop, *b = pop_swap;
+  for (int k = 0; k < nsteps; k++) {
+    if (xi_id == 0)
+      meta_step<SD2Q9>(geo, es, ord, G, 1., psi_kind, nullptr, a, b, n, psi, nullptr, nullptr, &t,
+                       &sp);
+    else
+      meta_step<SD3Q19>(geo, es, ord, G, 1., psi_kind, nullptr, a, b, n, psi, nullptr, nullptr,
+                        &t, &sp);
   }
   if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
   return 0;

@@ -31,6 +31,9 @@ def lib():
                                       C.c_double, C.c_int, C.c_int, C.c_void_p,
                                       C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, C.c_int,
                                       C.c_void_p]
+        L.cpuref_meta_run_mom.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
+                                          C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _dp,
+                                          _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
         L.cpuref_meta_moments.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
                                           C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p]
         L.cpuref_legacy_walls_run.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
@@ -148,6 +151,24 @@ def meta_run(st, estencil, G, tau, psi='exp', nsteps=1, mrt=None, walls=None):
     rc = lib().cpuref_meta_run(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, 1.0 / tau,
                                PSI_KIND[psi], int(st.aos), ptrs[0], ptrs[1], ptrs[2],
                                st.pop, st.pop_swap, st.n, st.psi, nsteps, wptr)
+    assert rc == 0
+
+
+def meta_run_mrt_moment(st, estencil, G, rates, psi='exp', nsteps=1):
+    """MainLoopMRT with the diagonal relaxation matrix diag(rates)
+    (idpy/LBM/MultiPhase.py:1311-1402), the collision evaluated in moment
+    space through the lattice monomials (same result as meta_run with the
+    dense matrices, up to rounding; the same sums in the same order as the
+    HIP kernel's moment-space collision)."""
+    from . import hermite_mrt
+    QE, Ei, Ew = _estencil(estencil)
+    t = hermite_mrt.moment_tables(st.xi)
+    s = np.ascontiguousarray(np.asarray(rates, dtype=np.float64).ravel())
+    assert s.size == (9 if st.xi == 'D2Q9' else 19)
+    rc = lib().cpuref_meta_run_mom(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, PSI_KIND[psi],
+                                   int(st.aos), len(t['pairs']), t['pairs'], t['mm'], t['odd'],
+                                   t['tm'], t['lt'], t['dinv'], t['ht'], t['mpv'], s, st.pop,
+                                   st.pop_swap, st.n, st.psi, nsteps)
     assert rc == 0
 
 

@@ -133,3 +133,51 @@ def mrt_matrices(stencil, S):
     Gg = Mp.inv() * (I - S / 2) * Mp
     f = lambda X: np.array(X.evalf(30).tolist(), dtype=np.float64)
     return f(A), f(B), f(Gg)
+
+
+MONOMIALS = {
+    'D2Q9': ((0, 0), (1, 0), (0, 1), (2, 0), (0, 2), (1, 1), (2, 1), (1, 2), (2, 2)),
+    'D3Q19': ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2),
+              (1, 1, 0), (1, 0, 1), (0, 1, 1), (2, 1, 0), (2, 0, 1), (1, 2, 0), (1, 0, 2),
+              (0, 2, 1), (0, 1, 2), (2, 2, 0), (2, 0, 2), (0, 2, 2)),
+}
+
+
+@lru_cache(maxsize=None)
+def moment_tables(stencil):
+    """Tables of the moment-space form of the same collision (diagonal S),
+    f* = f + W Mm^T Tm^T Dinv (Lt mg - md), from this module's bases:
+    Mm[k][q] = xi_q^e_k for the lattice monomials e_k, Tm = Mw Mm^-1,
+    Lt = Mw Mp^-1 (unit lower-triangular: Gram-Schmidt), Dinv the inverse
+    W-norms of the rows of Mw, the Hermite tuple of every row up to second
+    order, the first velocity of every opposite pair and MpV, the Mp
+    moments of the basis vectors w o {1, xi_a, xi_a xi_b (a <= b)}."""
+    xis = XI[stencil]
+    Q, D = len(xis), len(xis[0])
+    Mm = sp.Matrix([[sp.prod([sp.Integer(v[a]) ** e[a] for a in range(D)]) for v in xis]
+                    for e in MONOMIALS[stencil]])
+    Mw = worth_set(stencil)
+    Mp, tuples = invertible_set(stencil)
+    Tm = Mw * Mm.inv()
+    Lt = Mw * Mp.inv()
+    w = W[stencil]
+    Dg = [_wprod(Mw[k, :], Mw[k, :], w) for k in range(Q)]
+    opp = [xis.index(tuple(-c for c in v)) for v in xis]
+    pairs = [q for q in range(Q) if q < opp[q]]
+    ht = np.zeros((3, Q), dtype=np.int32)
+    for k, t in enumerate(tuples):
+        ht[0, k] = len(t) if len(t) <= 2 else 3
+        ht[1, k] = t[0] if 1 <= len(t) <= 2 else -1
+        ht[2, k] = t[1] if len(t) == 2 else -1
+    mons = [()] + [(a,) for a in range(D)] + [(a, a) for a in range(D)] + \
+        [(a, b) for a in range(D) for b in range(a + 1, D)]
+    Vb = sp.Matrix([[w[q] * sp.prod([sp.Integer(xis[q][a]) for a in m]) for q in range(Q)]
+                    for m in mons])
+    MpV = (Mp * Vb.T).T
+    f = lambda X: np.ascontiguousarray(np.array(X.evalf(30).tolist(), dtype=np.float64))
+    return {'pairs': np.array(pairs, dtype=np.int32),
+            'mm': np.ascontiguousarray(np.array(Mm.tolist(), dtype=np.int32)),
+            'odd': np.array([sum(e) % 2 for e in MONOMIALS[stencil]], dtype=np.int32),
+            'tm': f(Tm), 'lt': f(Lt),
+            'dinv': np.array([float(1 / d) for d in Dg], dtype=np.float64),
+            'ht': np.ascontiguousarray(ht), 'mpv': f(MpV)}

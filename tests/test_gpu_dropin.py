@@ -33,7 +33,9 @@ def test_legacy_class_c1_matches_oracle():
     cpuref.legacy_run(st, 'D2E4', -5.0 / 3.0, 1.0, nsteps=300)
     assert rel_err(sim.GetDensityField().ravel(), st.n) <= 1e-12
     assert rel_err(sim.sims_idpy_memory['psi'].D2H(), st.psi) <= 1e-12
-    assert rel_err(sim.sims_idpy_memory['u'].D2H(), st.u) <= 1e-11
+    e_u = rel_err(sim.sims_idpy_memory['u'].D2H(), st.u)
+    print('u rel err', e_u, flush=True)
+    assert e_u <= 1e-12
     assert rel_err(sim.sims_idpy_memory['pop'].D2H(), st.pop) <= 1e-12
     sim.End()
 
@@ -60,7 +62,9 @@ def test_meta_class_orderings(ordering):
     cpuref.meta_run(st, 'D2E4', -5.0 / 3.0, 1.0, nsteps=200)
     cpuref.meta_moments(st, 'D2E4', -5.0 / 3.0)
     assert rel_err(sim.GetDensityField().ravel(), st.n) <= 1e-12
-    assert rel_err(sim.GetVelocityField().ravel(), st.u) <= 1e-11
+    e_u = rel_err(sim.GetVelocityField().ravel(), st.u)
+    print('u rel err', e_u, flush=True)
+    assert e_u <= 1e-12
     pops = sim.GetPopulations()
     assert pops.shape == ((128, 128, 9) if ordering == 'cpu' else (9, 128, 128))
     assert rel_err(pops.ravel(), st.pop) <= 1e-12

@@ -106,10 +106,18 @@ def test_c1_droplet_full_length_meta_class(tenet):
     sim.MainLoop(list(range(0, 9001, 1000)))
     cpuref.meta_run(st, 'D2E4', G, 1.0, nsteps=9000)
     err4 = rel_err(sim.sims_idpy_memory['pop'].D2H().ravel(), st.pop)
+    # every field after ComputeMomentsMeta: n, psi and the Guo-shifted u
+    sim.ComputeMomentsMeta()
+    cpuref.meta_moments(st, 'D2E4', G)
+    e_n = rel_err(sim.GetDensityField().ravel(), st.n)
+    e_psi = rel_err(sim.sims_idpy_memory['psi'].D2H(), st.psi)
+    e_u = rel_err(sim.sims_idpy_memory['u'].D2H(), st.u)
     sim.End()
-    print('rel err after 1e3 steps', err3, 'after 1e4 steps', err4, flush=True)
+    print('rel err after 1e3 steps', err3, 'after 1e4 steps', err4, 'n', e_n, 'psi', e_psi,
+          'u', e_u, flush=True)
     assert err3 <= 1e-12
     assert err4 <= 1e-12
+    assert e_n <= 1e-12 and e_psi <= 1e-12 and e_u <= 1e-12
 
 
 def test_d3q19_mrt_lds_staged_long_run_matches_oracle(tenet):

@@ -70,7 +70,7 @@ def test_mainloopwalls_matches_oracle(tenet, xi, es, dims, nsteps):
     print('rel err pop', e_pop, 'psi', e_psi, 'n', e_n, 'u', e_u, 'mass drift',
           abs(m1 - m0) / m0, flush=True)
     assert e_pop <= 1e-12 and e_psi <= 1e-12 and e_n <= 1e-12
-    assert e_u <= 1e-11
+    assert e_u <= 1e-12
     assert abs(m1 - m0) <= 1e-13 * m0
 
 
@@ -136,7 +136,7 @@ def test_simple_walls_gravity_loop_matches_oracle(tenet, xi, es, dims, g, nsteps
     sim.End()
     print('rel err pop', e_pop, 'n', e_n, 'psi', e_psi, 'u', e_u, flush=True)
     assert e_pop <= 1e-12 and e_n <= 1e-12 and e_psi <= 1e-12
-    assert e_u <= 1e-11
+    assert e_u <= 1e-12
 
 
 @pytest.mark.parametrize('xi,es,dims,walled,nsteps', [('D2Q9', 'D2E4', (64, 37), False, 200),
@@ -179,7 +179,7 @@ def test_sc_forcing_loops_match_oracle(tenet, xi, es, dims, walled, nsteps):
     sim.End()
     print('rel err pop', e_pop, 'n', e_n, 'psi', e_psi, 'u', e_u, flush=True)
     assert e_pop <= 1e-12 and e_n <= 1e-12 and e_psi <= 1e-12
-    assert e_u <= 1e-11
+    assert e_u <= 1e-12
 
 
 def test_wall_loops_keep_walls_across_moment_callbacks(tenet):
@@ -296,4 +296,4 @@ def test_simple_loop_matches_oracle_legacy_kernels(tenet, xi, es, dims, nsteps):
     sim.End()
     print('rel err pop', e_pop, 'n', e_n, 'psi', e_psi, 'u', e_u, flush=True)
     assert e_pop <= 1e-12 and e_n <= 1e-12 and e_psi <= 1e-12
-    assert e_u <= 1e-11
+    assert e_u <= 1e-12

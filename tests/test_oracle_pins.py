@@ -365,6 +365,59 @@ def test_mrt_with_uniform_rates_is_bgk():
     assert rel_err(b.pop, a.pop) < 1e-14
 
 
+@pytest.mark.parametrize('name,es,dims', [('D2Q9', 'D2E4', (24, 21)),
+                                           ('D3Q19', 'D3E4', (11, 10, 9))])
+def test_mrt_moment_space_equals_dense(name, es, dims):
+    """The moment-space evaluation of the MRT collision (lattice monomials,
+    opposite-pair sums, Mw = Tm Mm, Mw = Lt Mp, Mw^-1 = W Mw^T Dinv) gives
+    the dense f* = A f + B f_eq + Gg F of the same diagonal S of distinct
+    rates to rounding over 30 steps. Planted errors fail: in D3Q19 the Guo
+    part without the Lt mixing (the forcing taken in the W-orthogonal basis:
+    for D2Q9 Lt = I, the two bases coincide), in D2Q9 a dropped c2^2 term of
+    the xxyy moment."""
+    import sympy as sp
+    from oracle import cpuref, hermite_mrt
+    Q = 9 if name == 'D2Q9' else 19
+    s = np.random.default_rng(3).uniform(0.6, 1.6, size=Q)
+    n0 = droplet_field(dims, seed=11)
+    V = n0.size
+    a = cpuref.State(name, dims)
+    cpuref.init_pop(a, n0, np.zeros(V * len(dims)))
+    b = cpuref.State(name, dims)
+    b.pop[:] = a.pop
+    c = cpuref.State(name, dims)
+    c.pop[:] = a.pop
+    S = sp.diag(*[sp.Rational(v).limit_denominator(10 ** 15) for v in s])
+    cpuref.meta_run(a, es, G, 1.0, nsteps=30, mrt=hermite_mrt.mrt_matrices(name, S))
+    cpuref.meta_run_mrt_moment(b, es, G, s, nsteps=30)
+    assert rel_err(b.pop, a.pop) < 1e-13
+    # the tables themselves: Tm Mm = Mw, Lt Mp = Mw, W-orthogonality
+    t = hermite_mrt.moment_tables(name)
+    Mw = np.array(hermite_mrt.worth_set(name).evalf(30).tolist(), dtype=float)
+    Mp = np.array(hermite_mrt.invertible_set(name)[0].evalf(30).tolist(), dtype=float)
+    assert np.abs(t['tm'] @ t['mm'] - Mw).max() < 1e-14
+    assert np.abs(t['lt'] @ Mp - Mw).max() < 1e-14
+    w = np.array([float(x) for x in hermite_mrt.W[name]])
+    assert np.abs(Mw @ np.diag(w) @ Mw.T @ np.diag(t['dinv']) - np.eye(Q)).max() < 1e-13
+    orig = hermite_mrt.moment_tables.__wrapped__(name)
+    if name == 'D3Q19':
+        assert np.abs(t['lt'] - np.eye(Q)).max() > 0.1
+        planted = dict(orig, lt=np.ascontiguousarray(np.eye(Q)))
+    else:
+        assert np.abs(t['lt'] - np.eye(Q)).max() == 0.0
+        tm = orig['tm'].copy()
+        tm[8, 0] = 0.0
+        planted = dict(orig, tm=np.ascontiguousarray(tm))
+    import oracle.hermite_mrt as hm
+    saved = hm.moment_tables
+    hm.moment_tables = lambda n: planted
+    try:
+        cpuref.meta_run_mrt_moment(c, es, G, s, nsteps=30)
+    finally:
+        hm.moment_tables = saved
+    assert rel_err(c.pop, a.pop) > 1e-9
+
+
 def test_mrt_product_basis_equals_oracle_basis():
     """The package derives the same bases from the 1D Hermite recurrence in
     Fractions; the oracle from derivatives of the Gaussian in sympy."""
