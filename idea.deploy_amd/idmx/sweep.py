@@ -14,12 +14,21 @@ Modes
   FP32 + FP64: any start, FP32 shifted populations to tol32, then the state
                is converted to FP64 and relaxed to tol (FP64 fixes the final
                accuracy)
-Fallback controller: a caller-supplied start that turns non-finite is reset
-to the cold start once; with grow_checks set, so is one whose residual stays
-above ten times its best value for that many consecutive checks. The growth
-rule is off by default: on a 128-point sweep it reset 40 learned starts and
-cost both time and converged points (the relaxation of a good start is not
-monotone). A cold start is never reset: its trajectory would repeat.
+Fallback controller (for caller-supplied starts; a cold start is never reset,
+its trajectory would repeat):
+  - a start whose total mass differs from the cold start's by more than
+    `mass_tol` (relative) describes another droplet and is replaced by the
+    cold start before the run; within it, the start is shifted uniformly to
+    the cold start's mass (the dynamics conserve mass, so that fixes the
+    steady state);
+  - a start that turns non-finite is reset to the cold start;
+  - a start whose residual grows is reset to the cold start: `grow_checks`
+    consecutive checks with max |dn| above ten times its best value so far
+    and above its value at the first check after the start. The second
+    condition keeps the transient rise of a good start (the relaxation is not
+    monotone) from triggering it.
+A reset domain gets a fresh step budget, so resets cost time but not
+converged points. SweepResult.reason records why each point fell back.
 """
 import ctypes as C
 import time
@@ -42,6 +51,10 @@ class SweepPoint:
     tau: float = 1.0
 
 
+# SweepResult.reason codes
+FB_NONE, FB_NONFINITE, FB_GROWTH, FB_MASS = 0, 1, 2, 3
+
+
 @dataclass
 class SweepResult:
     steps: np.ndarray
@@ -50,6 +63,12 @@ class SweepResult:
     wall_s: float
     gpu_ms: float
     history: list = field(default_factory=list)
+    reason: np.ndarray = None   # FB_* per point
+
+    def reset_counts(self):
+        r = self.reason if self.reason is not None else np.zeros(len(self.steps), dtype=int)
+        return {'nonfinite': int((r == FB_NONFINITE).sum()), 'growth': int((r == FB_GROWTH).sum()),
+                'mass': int((r == FB_MASS).sum())}
 
 
 def cold_start(p, W=2.0):
@@ -161,22 +180,27 @@ def _apply_launch(batch, active, launch, launch_small):
 
 def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks, hist,
            restart, label, plateau=False, launch=None, launch_small=None, plateau_checks=5,
-           progress=None):
+           progress=None, reason=None):
     """Runs the live domains of `batch` until each one's max |dn| < tol, or
-    until every live domain has run max_steps steps in this call. A domain
-    on a caller-supplied start (warm[i]) that turns non-finite, or (when
-    grow_checks is not None) whose residual stays above ten times its best
-    value for `grow_checks` consecutive checks, is handed to restart(i) (the
-    cold-start fallback) once. A cold domain that turns non-finite is dropped
-    as failed. With plateau=True (the FP32 phase) a domain also finishes
-    once its residual has not improved by 20 % for `plateau_checks` checks:
-    FP32 rounding sets a floor below which it cannot go, and the FP64 phase
-    takes over there.
-    launch / launch_small: see _apply_launch. progress(label, steps, live,
-    max |dn|) is called every 50 checks.
+    until every live domain has run max_steps steps since its (re)start in
+    this call. A domain on a caller-supplied start (warm[i]) that turns
+    non-finite, or (when grow_checks is not None) whose residual stays above
+    ten times its best value and above its value at the first check for
+    `grow_checks` consecutive checks, is handed to restart(i) (the cold-start
+    fallback) once, with a fresh step budget; reason[i] (if given) records
+    FB_NONFINITE or FB_GROWTH. A cold domain that turns non-finite is
+    dropped as failed. With plateau=True (the FP32 phase) a domain also
+    finishes once its residual has not improved by 20 % for `plateau_checks`
+    checks: FP32 rounding sets a floor below which it cannot go, and the
+    FP64 phase takes over there.
+    batch needs set_active, run(nsteps) -> GPU ms and check() -> max |dn|
+    per domain (NaN or inf for a non-finite state). launch / launch_small:
+    see _apply_launch. progress(label, steps, live, max |dn|) is called
+    every 50 checks.
     Returns (still active, failed, fell back, GPU ms)."""
     n = batch.n
     best = np.full(n, np.inf)
+    first = np.full(n, np.nan)   # residual at the first check after the start
     stale = np.zeros(n, dtype=int)
     grows = np.zeros(n, dtype=int)
     fell_back = np.zeros(n, dtype=bool)
@@ -193,7 +217,9 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
         ran[active] += check_every
         dn = batch.check()
         fin = np.isfinite(dn)
-        grows = np.where(active & fin & (dn > 10 * best) & (dn > tol), grows + 1, 0)
+        first = np.where(active & fin & np.isnan(first), dn, first)
+        grows = np.where(active & fin & (dn > 10 * best) & (dn > first) & (dn > tol),
+                         grows + 1, 0)
         stale = np.where(active & fin & (dn > 0.8 * best), stale + 1, 0)
         best = np.where(active & fin, np.minimum(best, dn), best)
         done = active & fin & (dn < tol)
@@ -210,11 +236,15 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
         changed = (done | dead).any()
         active = active & ~done & ~dead
         for i in np.nonzero(bad & warm)[0]:
+            if reason is not None:
+                reason[i] = FB_NONFINITE if not fin[i] else FB_GROWTH
             restart(i)
             warm[i] = False
             fell_back[i] = True
             grows[i] = stale[i] = 0
             best[i] = np.inf
+            first[i] = np.nan
+            ran[i] = 0
         if not active.any() or (ran[active] >= max_steps).all():
             break
         if changed:
@@ -223,10 +253,43 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
     return active, failed, fell_back, gpu_ms
 
 
+def screen_starts(points, start, mass_tol=0.05):
+    """The initial densities of a sweep: the cold start where start is None
+    (or start[i] is None), else start[i], unless it is not finite or its
+    mass differs from the cold start's by more than mass_tol (relative):
+    then the cold start, recorded as a fallback. Returns (densities,
+    warm flags, fell-back flags, FB_* reasons, cold-start masses)."""
+    n = len(points)
+    warm = np.array([start is not None and start[i] is not None for i in range(n)], dtype=bool)
+    init = [cold_start(p) if not warm[i] else np.asarray(start[i], dtype=np.float64).ravel()
+            for i, p in enumerate(points)]
+    for i, p in enumerate(points):
+        if init[i].size != p.L * p.L:
+            raise ValueError('run_sweep: start %d has %d values for a %dx%d domain'
+                             % (i, init[i].size, p.L, p.L))
+    fb = np.zeros(n, dtype=bool)
+    reason = np.zeros(n, dtype=int)
+    mass0 = np.array([float(np.sum(cold_start(p))) for p in points])
+    for i, p in enumerate(points):
+        if not warm[i]:
+            continue
+        if not np.isfinite(init[i]).all():
+            bad = FB_NONFINITE
+        elif abs(float(np.sum(init[i])) - mass0[i]) > mass_tol * mass0[i]:
+            bad = FB_MASS
+        else:
+            continue
+        init[i] = cold_start(p)
+        warm[i] = False
+        fb[i] = True
+        reason[i] = bad
+    return init, warm, fb, reason, mass0
+
+
 def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
-              check_every=200, max_steps=400000, grow_checks=None, estencil='E4',
+              check_every=200, max_steps=400000, grow_checks=5, estencil='E4',
               launch=None, launch32=None, rho0=1.0, symmetric=True, launch_small=None,
-              launch32_small=None, progress=None):
+              launch32_small=None, progress=None, mass_tol=0.05):
     """Relax every point to steady state; returns SweepResult and the final
     FP64 densities. start: None (cold) or a list of initial densities (None
     entries start cold). mode: 'fp64' (FP64 only) or 'fp32+fp64'.
@@ -237,25 +300,14 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     symmetric: the points are centred droplets, whose steady state is
     symmetric about the domain centre; warm starts and the FP32 -> FP64
     hand-over are projected onto symmetric fields (a broken symmetry makes
-    the droplet drift towards a symmetric lattice position for ~1e5 steps)."""
+    the droplet drift towards a symmetric lattice position for ~1e5 steps).
+    grow_checks / mass_tol: the fallback rules of the module docstring
+    (grow_checks=None switches the growth rule off)."""
     n = len(points)
     steps = np.zeros(n, dtype=np.int64)
     hist = []
     fail_all = np.zeros(n, dtype=bool)
-    warm = np.array([start is not None and start[i] is not None for i in range(n)])
-    init = [cold_start(p) if not warm[i] else np.asarray(start[i], dtype=np.float64).ravel()
-            for i, p in enumerate(points)]
-    for i, p in enumerate(points):
-        if init[i].size != p.L * p.L:
-            raise ValueError('run_sweep: start %d has %d values for a %dx%d domain'
-                             % (i, init[i].size, p.L, p.L))
-    fb_all = np.zeros(n, dtype=bool)
-    for i, p in enumerate(points):
-        if warm[i] and not np.isfinite(init[i]).all():   # unusable warm start
-            init[i] = cold_start(p)
-            warm[i] = False
-            fb_all[i] = True
-    mass0 = np.array([float(np.sum(a)) for a in init])
+    init, warm, fb_all, reason, mass0 = screen_starts(points, start, mass_tol)
     t0 = time.perf_counter()
     b64 = Batch(tenet, points, 'fp64', estencil)
     if launch:
@@ -273,8 +325,8 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
             b32.set_launch(*launch32)
         for i in range(n):
             b32.set_density(i, init[i])
-            if symmetric and warm[i]:
-                b32.symmetrize(i)
+            if warm[i]:
+                _shift_to_mass(b32, i, mass0, symmetric)
 
         def restart32(i):
             n0 = cold_start(points[i])
@@ -284,7 +336,7 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
         _, failed, fb, ms = _relax(b32, np.ones(n, dtype=bool), warm, steps, tol32, check_every,
                                    max_steps, grow_checks, hist, restart32, 'fp32',
                                    plateau=True, launch=launch32, launch_small=launch32_small,
-                                   progress=progress)
+                                   progress=progress, reason=reason)
         gpu_ms += ms
         fb_all |= fb
         # FP32 rounding leaves each domain a small net momentum that FP64
@@ -302,13 +354,14 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     elif mode == 'fp64':
         for i in range(n):
             b64.set_density(i, init[i])
-            if symmetric and warm[i]:
-                b64.symmetrize(i)
+            if warm[i]:
+                _shift_to_mass(b64, i, mass0, symmetric)
     else:
         raise ValueError('mode is fp64 or fp32+fp64')
     rem, failed, fb, ms = _relax(b64, np.ones(n, dtype=bool), warm, steps, tol, check_every,
                                  max_steps, grow_checks, hist, restart64, 'fp64',
-                                 launch=launch, launch_small=launch_small, progress=progress)
+                                 launch=launch, launch_small=launch_small, progress=progress,
+                                 reason=reason)
     gpu_ms += ms
     fb_all |= fb
     fail_all |= failed
@@ -316,7 +369,17 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     b64.close()
     wall = time.perf_counter() - t0
     return SweepResult(steps=steps, converged=~rem & ~fail_all, fallback=fb_all, wall_s=wall,
-                       gpu_ms=gpu_ms, history=hist), fields
+                       gpu_ms=gpu_ms, history=hist, reason=reason), fields
+
+
+def _shift_to_mass(batch, i, mass0, symmetric):
+    """Sets domain i's density to the cold start's mass by a uniform shift
+    (with symmetric, after projecting it onto the centre-symmetric fields)."""
+    if symmetric:
+        batch.symmetrize(i, mass=mass0)
+    else:
+        n = batch.get_density(i)
+        batch.set_density(i, n + (mass0[i] - n.sum()) / n.size)
 
 
 def is_centred_droplet(n, tol=2.0):
