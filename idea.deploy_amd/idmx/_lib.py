@@ -56,6 +56,8 @@ _SIGS = [
     ('idmx_max_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, c_double_p]),
     ('idmx_max_abs_diff_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                         c_double_p]),
+    ('idmx_max_abs_diff_f64_noalloc', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                                c_double_p]),
     ('idmx_stream_copy_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                        c_double_p]),
     ('idmx_check_u', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int,

@@ -141,6 +141,19 @@ int idmx_max_f64(idmx_ctx_t ctx, const double* x, int64_t n, double* out);
 int idmx_max_abs_diff_f64(idmx_ctx_t ctx, const double* a, const double* b,
                           int64_t n, double* out);
 
+/* idmx_max_abs_diff_f64 for callers that cannot allocate device memory
+ * (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. It takes a HIP stream
+ * (hipStream_t as void*, NULL for the null stream) of the current device
+ * instead of a context and calls no hipMalloc, hipFree or allocating
+ * routine: its partial results (at most 1024, 8 KiB) go to a static device
+ * array, one copy per GPU, held until the process ends and looked up at the
+ * first call on each device. Any n >= 0 is accepted (the reduction grid is
+ * capped at 1024 workgroups). Calls that overlap on one GPU share that
+ * array and race: serialise them. Blocks until the result is on the host. */
+int idmx_max_abs_diff_f64_noalloc(void* stream, const double* a, const double* b,
+                                  int64_t n, double* out);
+
 /* K_CenterOfMass + IdpyMemory.Sum (idpy/LBM/MultiPhase.py:108-137) over a
  * float64 density field of L0 x L1 x L2 sites (x fastest; L2 = 1 in 2D):
  * out[0] = sum n, out[1 + d] = sum n x_d / sum n, coordinates not
