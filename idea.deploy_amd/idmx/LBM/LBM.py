@@ -51,8 +51,6 @@ class ShanChenMultiPhase(_ShanChenCommon):
         self.init_status['u'] = True
 
     def InitRadialInterface(self, n_g, n_l, R, W=1, full_flag=True):
-        """K_InitRadialInterface, tanh((r - R)/W) (LBM.py:1786-1826)."""
-        self.sims_vars.update({'init_type': 'radial', 'n_g': n_g, 'n_l': n_l,
-                               'full_flag': full_flag, 'R': R})
-        self._set_n_u0(self._radial(n_g, n_l, R, W, full_flag))
-        self.InitPopulations()
+        """K_InitRadialInterface, tanh((r - R)/W) (LBM.py:1786-1826), on the
+        device."""
+        self._init_radial(n_g, n_l, R, W, full_flag)

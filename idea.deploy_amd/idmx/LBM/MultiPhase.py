@@ -225,20 +225,15 @@ class ShanChenMultiPhase(_ShanChenCommon):
         self.init_status['u'] = True
 
     def InitRadialInterface(self, n_g, n_l, R, full_flag=True):
-        """K_InitRadialInterface, tanh(r - R) (MultiPhaseKernels.py:686-727)."""
-        self.sims_vars.update({'init_type': 'radial', 'n_g': n_g, 'n_l': n_l,
-                               'full_flag': full_flag, 'R': R})
-        self._set_n_u0(self._radial(n_g, n_l, R, 1.0, full_flag))
-        self.InitPopulations()
+        """K_InitRadialInterface, tanh(r - R) (MultiPhaseKernels.py:686-727),
+        on the device."""
+        self._init_radial(n_g, n_l, R, 1.0, full_flag)
 
     def InitSingleFlatInterface(self, n_g, n_l, direction=0, full_flag=True):
         """K_InitSingleFlatInterface with F_NSingleFlatProfile
-        (MultiPhaseKernels.py:45-53, 650-684)."""
+        (MultiPhaseKernels.py:45-53, 650-684), on the device."""
         self.sims_vars.update({'init_type': 'flat', 'n_g': n_g, 'n_l': n_l,
                                'full_flag': full_flag, 'direction': direction})
-        x = self._coords()[direction]
-        x0 = self.sims_vars['dim_center'][direction]
-        ff = int(bool(full_flag))
-        dn = ff * (n_l - n_g) + (1 - ff) * (n_g - n_l)
-        self._set_n_u0(n_g + dn * (np.tanh(x - x0) + 1.) * 0.5)
+        self._device_init('idmx_init_single_flat', float(n_g), float(n_l), int(direction),
+                          int(bool(full_flag)))
         self.InitPopulations()

@@ -13,6 +13,7 @@ from functools import reduce
 import numpy as np
 import sympy as sp
 
+from .. import _lib
 from ..IdpyCode import GetParamsClean, GetTenet, HIP_T
 from ..IdpyCode import IdpyMemory
 from ..IdpyCode import IdpySims as IdpySimsIO
@@ -364,12 +365,6 @@ class _ShanChenCommon(RootLB):
         m['EW_list'] = IdpyMemory.OnDevice(self.sims_vars['EW_list'], t)
 
     # -- initial fields (host-side geometry, then H2D) -----------------------
-    def _coords(self):
-        sizes = self.sims_vars['dim_sizes']
-        grids = np.meshgrid(*[np.arange(s) for s in sizes], indexing='ij')
-        # flatten in lexicographic order g = x + y*L0 (+ z*L0*L1)
-        return [g.transpose(tuple(range(len(sizes)))[::-1]).ravel() for g in grids]
-
     def _set_n_u0(self, n):
         m = self.sims_idpy_memory
         m['n'].H2D(np.asarray(n, dtype=np.float64))
@@ -385,45 +380,44 @@ class _ShanChenCommon(RootLB):
         self.engine.init_pop(m['pop'], m['n'], m['u'])
         self.init_status['pop'] = True
 
+    def _device_init(self, fn, *args):
+        """Runs one of the initial-interface kernels (idmx_init_*) on n and u,
+        then marks both initialised."""
+        import ctypes as C
+        m = self.sims_idpy_memory
+        sizes = [int(v) for v in self.sims_vars['dim_sizes']]
+        dims = (C.c_int * 3)(*(sizes + [1] * (3 - len(sizes))))
+        _lib.call(fn, self.tenet.handle, m['n'].ptr, m['u'].ptr, len(sizes), dims, *args)
+        self.tenet.Sync()
+        self.init_status['n'] = True
+        self.init_status['u'] = True
+
     def InitFlatInterface(self, n_g, n_l, width, direction=0, full_flag=True):
         """K_InitFlatInterface (idpy/LBM/LBM.py:1745-1784) with
-        F_NFlatProfilePeriodic (idpy/LBM/MultiPhaseKernels.py:55-74)."""
+        F_NFlatProfilePeriodic (idpy/LBM/MultiPhaseKernels.py:55-74), on the
+        device (idmx_init_flat)."""
         self.sims_vars.update({'init_type': 'flat', 'n_g': n_g, 'n_l': n_l,
                                'full_flag': full_flag, 'width': width, 'direction': direction})
-        x = self._coords()[direction]
-        size = int(self.sims_vars['dim_sizes'][direction])
-        x0 = int(self.sims_vars['dim_center'][direction])
-        xm = size // 2
-        if x0 - xm >= 0:
-            xp = np.where(x < x0 - xm, x - (x0 - xm) + size, x - (x0 - xm))
-        else:
-            xp = np.where(x < x0 + xm, x - (x0 + xm) + size, x - (x0 + xm))
-        prof = np.tanh(xp - (xm - 0.5 * width)) - np.tanh(xp - (xm + 0.5 * width))
-        ff = int(bool(full_flag))
-        dn = ff * (n_l - n_g) + (1 - ff) * (n_g - n_l)
-        self._set_n_u0(0.5 * (n_g + n_l) + 0.5 * dn * (prof - 1.))
+        self._device_init('idmx_init_flat', float(n_g), float(n_l), float(width),
+                          int(direction), int(bool(full_flag)))
         self.InitPopulations()
 
     def InitCylinderInterface(self, n_g, n_l, R, direction=0, full_flag=True):
-        """K_InitCylinderInterface (idpy/LBM/LBM.py:1828-1874)."""
+        """K_InitCylinderInterface (idpy/LBM/LBM.py:1828-1874), on the device
+        (idmx_init_cylinder)."""
         self.sims_vars.update({'init_type': 'cylinder', 'n_g': n_g, 'n_l': n_l,
                                'full_flag': full_flag, 'R': R, 'direction': direction})
-        xs = self._coords()
-        c = self.sims_vars['dim_center']
-        r2 = sum((c[d] - xs[d]) ** 2 for d in range(len(xs)) if d != direction)
-        r = np.sqrt(r2.astype(np.float64))
-        ff = int(bool(full_flag))
-        dn = ff * (n_l - n_g) + (1 - ff) * (n_g - n_l)
-        self._set_n_u0(0.5 * (n_g + n_l) - 0.5 * dn * np.tanh(r - R))
+        self._device_init('idmx_init_cylinder', float(n_g), float(n_l), float(R),
+                          int(direction), int(bool(full_flag)))
         self.InitPopulations()
 
-    def _radial(self, n_g, n_l, R, W, full_flag):
-        xs = self._coords()
-        c = self.sims_vars['dim_center']
-        r = np.sqrt(sum((c[d] - xs[d]) ** 2 for d in range(len(xs))).astype(np.float64))
-        ff = int(bool(full_flag))
-        dn = ff * (n_l - n_g) + (1 - ff) * (n_g - n_l)
-        return 0.5 * (n_g + n_l) - 0.5 * dn * np.tanh((r - R) / W)
+    def _init_radial(self, n_g, n_l, R, W, full_flag):
+        """K_InitRadialInterface on the device (idmx_init_radial)."""
+        self.sims_vars.update({'init_type': 'radial', 'n_g': n_g, 'n_l': n_l,
+                               'full_flag': full_flag, 'R': R})
+        self._device_init('idmx_init_radial', float(n_g), float(n_l), float(R), float(W),
+                          int(bool(full_flag)))
+        self.InitPopulations()
 
     def SetDensityField(self, n):
         """Sets n (array in (z,) y, x order or flat lexicographic), u = 0 and

@@ -58,6 +58,14 @@ _SIGS = [
                                         c_double_p]),
     ('idmx_max_abs_diff_f64_noalloc', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                                 c_double_p]),
+    ('idmx_init_radial', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_int_p,
+                                   C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
+    ('idmx_init_flat', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_int_p,
+                                 C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]),
+    ('idmx_init_cylinder', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_int_p,
+                                     C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]),
+    ('idmx_init_single_flat', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_int_p,
+                                        C.c_double, C.c_double, C.c_int, C.c_int]),
     ('idmx_stream_copy_f64', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                        c_double_p]),
     ('idmx_check_u', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int,

@@ -35,7 +35,8 @@ SOURCES = [('runtime.hip', NOFMA), ('sc_d2q9.hip', NOFMA), ('sc_d2q9_f32.hip', [
            ('sc_d3q19_e4.hip', NOFMA), ('sc_d3q19_e4_f32.hip', []),
            ('sc_d3q19_e6.hip', NOFMA), ('sc_d3q19_e6_f32.hip', []),
            ('sc_d3q19_e8.hip', NOFMA), ('sc_d3q19_e8_f32.hip', []),
-           ('batch.hip', NOFMA), ('batch_f32.hip', []), ('nn.hip', []), ('noalloc.hip', NOFMA)]
+           ('batch.hip', NOFMA), ('batch_f32.hip', []), ('nn.hip', []), ('noalloc.hip', NOFMA),
+           ('init_fields.hip', NOFMA)]
 
 
 def _run(cmd):

@@ -174,6 +174,30 @@ int idmx_stream_copy_f64(idmx_ctx_t ctx, double* dst, const double* src,
 int idmx_check_u(idmx_ctx_t ctx, const double* u, double* old_u, int64_t V,
                  int D, double* sum_delta, double* max_u2);
 
+/* ---- initial interfaces: n from the lattice position, u = 0 -----------
+ * n and u are float64 device arrays of V and D*V values on a D-dimensional
+ * lattice dims[0..D-1] (x fastest), centred on dim_center = dims/2
+ * (idpy/LBM/LBM.py:422-423); full_flag != 0 puts the liquid n_l inside
+ * (delta_n = n_l - n_g), 0 the gas. */
+/* K_InitRadialInterface: n = (n_g+n_l)/2 - delta_n/2 tanh((r - R)/W), r the
+ * distance to the centre (legacy class idpy/LBM/LBM.py:1786-1826; the
+ * Meta class's kernel, idpy/LBM/MultiPhaseKernels.py:686-727, is W = 1). */
+int idmx_init_radial(idmx_ctx_t ctx, double* n, double* u, int D, const int* dims,
+                     double n_g, double n_l, double R, double W, int full_flag);
+/* K_InitFlatInterface (idpy/LBM/LBM.py:1745-1784) with
+ * F_NFlatProfilePeriodic (idpy/LBM/MultiPhaseKernels.py:55-74): a periodic
+ * slab of width `width` centred along `direction`. */
+int idmx_init_flat(idmx_ctx_t ctx, double* n, double* u, int D, const int* dims,
+                   double n_g, double n_l, double width, int direction, int full_flag);
+/* K_InitCylinderInterface (idpy/LBM/LBM.py:1828-1874): the radial profile
+ * (W = 1) of the distance in the plane orthogonal to `direction`. */
+int idmx_init_cylinder(idmx_ctx_t ctx, double* n, double* u, int D, const int* dims,
+                       double n_g, double n_l, double R, int direction, int full_flag);
+/* K_InitSingleFlatInterface (idpy/LBM/MultiPhaseKernels.py:650-684):
+ * n = n_g + delta_n (tanh(x_dir - c_dir) + 1)/2. */
+int idmx_init_single_flat(idmx_ctx_t ctx, double* n, double* u, int D, const int* dims,
+                          double n_g, double n_l, int direction, int full_flag);
+
 /* ---- Shan-Chen simulation -------------------------------------------- */
 int idmx_sc_create(idmx_ctx_t ctx, const idmx_sc_desc* desc, idmx_sc_t* out);
 int idmx_sc_destroy(idmx_sc_t sim);

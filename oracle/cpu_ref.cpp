@@ -41,7 +41,9 @@
 //   * ComputeMomentsMeta (idpy/LBM/MultiPhase.py:1565-1662,
 //     K_ComputeVelocityAfterForceSCMPMeta MultiPhaseKernelsMeta.py:1077-1219)
 //   * InitPopulations (K_InitPopulations idpy/LBM/LBM.py:1700-1743)
-//   * InitRadialInterface (legacy, idpy/LBM/LBM.py:1786-1826)
+//   * InitRadialInterface (legacy, idpy/LBM/LBM.py:1786-1826), InitFlatInterface
+//     (:1745-1784), InitCylinderInterface (:1828-1874) and the Meta class's
+//     InitSingleFlatInterface (idpy/LBM/MultiPhaseKernels.py:650-684)
 //
 // Constants are full doubles ("exact" set): the legacy class reads W_list and
 // EW_list from memory as doubles (idpy/LBM/LBM.py:1596-1607), and the stencil
@@ -856,6 +858,68 @@ int cpuref_init_radial(int D, const int* L, double n_g, double n_l, double R,
     for (int d = 0; d < D; d++) dist += (c[d] - p[d]) * (c[d] - p[d]);
     const double r = sqrt(dist);
     n[g] = 0.5 * (n_g + n_l) - 0.5 * delta_n * tanh((r - R) / W);
+    for (int d = 0; d < D; d++) u[g + d * geo.V] = 0.;
+  }
+  return 0;
+}
+
+// K_InitFlatInterface (idpy/LBM/LBM.py:1745-1784) with
+// F_NFlatProfilePeriodic (idpy/LBM/MultiPhaseKernels.py:55-74); u = 0.
+int cpuref_init_flat(int D, const int* L, double n_g, double n_l, double width, int dir,
+                     int full_flag, double* n, double* u) {
+  Geo geo;
+  if (!make_geo(D, L, &geo) || dir < 0 || dir >= D) return -1;
+  const int c = geo.L[dir] / 2, size = geo.L[dir];
+  const double delta_n = full_flag * (n_l - n_g) + (1 - full_flag) * (n_g - n_l);
+  for (int64_t g = 0; g < geo.V; g++) {
+    int p[3];
+    geo.pos(g, p);
+    const int x = p[dir], xm = size / 2;
+    int xp = 0;
+    const int delta = c - xm;
+    if (delta >= 0) xp = x < c - xm ? x - (c - xm) + size : x - (c - xm);
+    else xp = x < c + xm ? x - (c + xm) + size : x - (c + xm);
+    const double prof =
+        tanh((double)(xp - (xm - 0.5 * width))) - tanh((double)(xp - (xm + 0.5 * width)));
+    n[g] = 0.5 * (n_g + n_l) + 0.5 * delta_n * (prof - 1.);
+    for (int d = 0; d < D; d++) u[g + d * geo.V] = 0.;
+  }
+  return 0;
+}
+
+// K_InitCylinderInterface (idpy/LBM/LBM.py:1828-1874); u = 0.
+int cpuref_init_cylinder(int D, const int* L, double n_g, double n_l, double R, int dir,
+                         int full_flag, double* n, double* u) {
+  Geo geo;
+  if (!make_geo(D, L, &geo) || dir < 0 || dir >= D) return -1;
+  int c[3] = {geo.L[0] / 2, geo.L[1] / 2, geo.L[2] / 2};
+  const double delta_n = full_flag * (n_l - n_g) + (1 - full_flag) * (n_g - n_l);
+  for (int64_t g = 0; g < geo.V; g++) {
+    int p[3];
+    geo.pos(g, p);
+    p[dir] = 0;
+    int cp[3] = {c[0], c[1], c[2]};
+    cp[dir] = 0;
+    double dist = 0;
+    for (int d = 0; d < D; d++) dist += (cp[d] - p[d]) * (cp[d] - p[d]);
+    n[g] = 0.5 * (n_g + n_l) - 0.5 * delta_n * tanh(sqrt(dist) - R);
+    for (int d = 0; d < D; d++) u[g + d * geo.V] = 0.;
+  }
+  return 0;
+}
+
+// K_InitSingleFlatInterface (idpy/LBM/MultiPhaseKernels.py:650-684) with
+// F_NSingleFlatProfile (:45-53); u = 0.
+int cpuref_init_single_flat(int D, const int* L, double n_g, double n_l, int dir, int full_flag,
+                            double* n, double* u) {
+  Geo geo;
+  if (!make_geo(D, L, &geo) || dir < 0 || dir >= D) return -1;
+  const int c = geo.L[dir] / 2;
+  const double delta_n = full_flag * (n_l - n_g) + (1 - full_flag) * (n_g - n_l);
+  for (int64_t g = 0; g < geo.V; g++) {
+    int p[3];
+    geo.pos(g, p);
+    n[g] = n_g + delta_n * ((tanh((double)(p[dir] - c)) + 1.) * 0.5);
     for (int d = 0; d < D; d++) u[g + d * geo.V] = 0.;
   }
   return 0;

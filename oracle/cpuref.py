@@ -46,6 +46,12 @@ def lib():
         L.cpuref_init_pop.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp]
         L.cpuref_init_radial.argtypes = [C.c_int, _ip, C.c_double, C.c_double,
                                          C.c_double, C.c_double, _dp, _dp]
+        L.cpuref_init_flat.argtypes = [C.c_int, _ip, C.c_double, C.c_double, C.c_double,
+                                       C.c_int, C.c_int, _dp, _dp]
+        L.cpuref_init_cylinder.argtypes = [C.c_int, _ip, C.c_double, C.c_double, C.c_double,
+                                           C.c_int, C.c_int, _dp, _dp]
+        L.cpuref_init_single_flat.argtypes = [C.c_int, _ip, C.c_double, C.c_double, C.c_int,
+                                              C.c_int, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -90,6 +96,33 @@ def init_radial(dims, n_g, n_l, R, W):
     u = np.zeros(V * len(dims))
     rc = lib().cpuref_init_radial(len(dims), _L(dims), n_g, n_l, R, W, n, u)
     assert rc == 0
+    return n, u
+
+
+def init_flat(dims, n_g, n_l, width, direction=0, full_flag=True):
+    """K_InitFlatInterface (idpy/LBM/LBM.py:1745-1784)."""
+    V = int(np.prod(dims))
+    n, u = np.zeros(V), np.zeros(V * len(dims))
+    assert lib().cpuref_init_flat(len(dims), _L(dims), n_g, n_l, width, direction,
+                                  int(bool(full_flag)), n, u) == 0
+    return n, u
+
+
+def init_cylinder(dims, n_g, n_l, R, direction=0, full_flag=True):
+    """K_InitCylinderInterface (idpy/LBM/LBM.py:1828-1874)."""
+    V = int(np.prod(dims))
+    n, u = np.zeros(V), np.zeros(V * len(dims))
+    assert lib().cpuref_init_cylinder(len(dims), _L(dims), n_g, n_l, R, direction,
+                                      int(bool(full_flag)), n, u) == 0
+    return n, u
+
+
+def init_single_flat(dims, n_g, n_l, direction=0, full_flag=True):
+    """K_InitSingleFlatInterface (idpy/LBM/MultiPhaseKernels.py:650-684)."""
+    V = int(np.prod(dims))
+    n, u = np.zeros(V), np.zeros(V * len(dims))
+    assert lib().cpuref_init_single_flat(len(dims), _L(dims), n_g, n_l, direction,
+                                         int(bool(full_flag)), n, u) == 0
     return n, u
 
 
