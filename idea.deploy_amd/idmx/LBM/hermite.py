@@ -277,3 +277,36 @@ def basis_moments(name):
     Mp, _ = invertible_set(name)
     mom = lambda M: [[sum(M[k][q] * v[q] for q in range(Q)) for k in range(Q)] for v in V]
     return mom(Mw), mom(Mp)
+
+
+@lru_cache(maxsize=None)
+def moment_polys(name, worth=True, root_xi_sym='\\xi'):
+    """The moment basis as polynomials in the symbols root_xi_sym_0..D-1
+    (sympy): the Hermite polynomials c_s^n H^(n)(xi / c_s) of the invertible
+    set, or with worth=True their Gram-Schmidt combinations (the same
+    coefficients as the rows of the W-orthogonal basis)."""
+    import sympy as sp
+    _, tuples = invertible_set(name)
+    _, _, D = _lattice(name)
+    xi = [sp.Symbol('%s_%d' % (root_xi_sym, a)) for a in range(D)]
+    c2 = sp.Rational(C2.numerator, C2.denominator)
+
+    def h1(k, x):
+        a, b = sp.Integer(1), x
+        if k == 0:
+            return a
+        for j in range(1, k):
+            a, b = b, sp.expand(x * b - j * c2 * a)
+        return b
+
+    raw = [sp.expand(sp.Mul(*[h1(t.count(a), xi[a]) for a in range(D)])) for t in tuples]
+    if not worth:
+        return tuple(raw)
+    _, coeffs = worth_set(name)
+    out = []
+    for i, p in enumerate(raw):
+        q = p
+        for j, c in enumerate(coeffs[i]):
+            q = q + sp.Rational(c.numerator, c.denominator) * out[j]
+        out.append(sp.expand(q))
+    return tuple(out)

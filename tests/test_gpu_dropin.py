@@ -179,3 +179,53 @@ def test_compute_moments_unshifted_velocity(cls):
     assert rel_err(n_d, n_h) <= 1e-15
     assert np.abs(u_d - u_h).max() <= 1e-15 * max(1.0, np.abs(u_h).max()) + 1e-17
     assert np.abs(u_h).max() > 1e-6
+
+
+@pytest.mark.parametrize('kind', ['diagonal', 'coupled'])
+def test_mainloopmrt_with_relaxation_matrix(kind):
+    """MainLoopMRT through the Meta class, with S built by RelaxationMatrix
+    as a notebook does: symbolic rates fixed by omega_syms_vals (diagonal:
+    moment-space kernel, against the oracle's moment-space evaluation and its
+    dense matrices) or a matrix coupling two moments (dense tables, against
+    the oracle's dense matrices)."""
+    from helpers import droplet_field
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.Collisions import RelaxationMatrix
+    from idmx.LBM.Equilibria import HermiteEquilibria
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    from oracle import cpuref, hermite_mrt
+    dims = (26, 21, 19)
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=dims, xi_stencil=XIStencils['D3Q19'],
+                             f_stencil=FStencils['D3E4'], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=-5.0 / 3.0, tau=1.0)
+    rm = RelaxationMatrix(HermiteEquilibria(xi_stencil=XIStencils['D3Q19']))
+    rates = np.random.default_rng(8).uniform(0.7, 1.5, 19)
+    om = sp.symbols('omega_0:19')
+    S = rm.GetDiagonalMatrix(list(om))
+    vals = [(o, sp.Float(v, 17)) for o, v in zip(om, rates)]
+    if kind == 'coupled':
+        ms = rm.GetMoments()
+        S = S + rm.SetOmegasByMoment({(ms[4], ms[7]): sp.Float(0.05, 17)})
+    n0 = droplet_field(dims, seed=41)
+    sim.SetDensityField(n0)
+    sim.MainLoopMRT([0, 20, 60], relaxation_matrix=S, omega_syms_vals=vals)
+    f = sim.sims_idpy_memory['pop'].D2H().ravel()
+    sim.End()
+    Snum = np.array(S.subs(vals).evalf(), dtype=np.float64)
+    Ssym = sp.Matrix(19, 19, lambda i, j: sp.Rational(Snum[i, j]).limit_denominator(10 ** 15))
+    st = cpuref.State('D3Q19', dims)
+    cpuref.init_pop(st, n0, np.zeros(3 * n0.size))
+    cpuref.meta_run(st, 'D3E4', -5.0 / 3.0, 1.0, nsteps=60,
+                    mrt=hermite_mrt.mrt_matrices('D3Q19', Ssym))
+    e_dense = rel_err(f, st.pop)
+    e_mom = None
+    if kind == 'diagonal':
+        st2 = cpuref.State('D3Q19', dims)
+        cpuref.init_pop(st2, n0, np.zeros(3 * n0.size))
+        cpuref.meta_run_mrt_moment(st2, 'D3E4', -5.0 / 3.0, np.diag(Snum), nsteps=60)
+        e_mom = rel_err(f, st2.pop)
+        assert e_mom <= 1e-12
+    print(kind, 'MainLoopMRT rel err vs dense oracle', e_dense, 'vs moment oracle', e_mom,
+          flush=True)
+    assert e_dense <= 1e-12

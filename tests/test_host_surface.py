@@ -32,3 +32,38 @@ def test_index_position_round_trip(dims):
         lex = pos[0] + dims[0] * (pos[1] + (dims[1] * pos[2] if len(dims) == 3 else 0))
         assert lex == g
         assert IndexFromPos(pos, strides) == g
+
+
+def test_relaxation_matrix_moments_are_the_bases():
+    """RelaxationMatrix (idpy/LBM/Collisions.py:41-84) on the drop-in: its
+    moment polynomials evaluated on the velocity set are the rows of the
+    W-orthogonal basis (default) and of the non-orthogonal invertible basis
+    (WOrth_flag=False), exactly; GetIndexFromMoment / SetOmegasByMoment /
+    GetDiagonalMatrix index rows in that order."""
+    import sympy as sp
+    from fractions import Fraction
+    from idmx.LBM import hermite
+    from idmx.LBM.Collisions import RelaxationMatrix
+    from idmx.LBM.Equilibria import HermiteEquilibria
+    from idmx.LBM.stencils import XIStencils
+    for name in ('D2Q9', 'D3Q19'):
+        eq = HermiteEquilibria(xi_stencil=XIStencils[name])
+        xis = XIStencils[name]['XIs']
+        for worth, rows in ((True, hermite.worth_set(name)[0]),
+                            (False, hermite.invertible_set(name)[0])):
+            rm = RelaxationMatrix(eq, WOrth_flag=worth)
+            ms = rm.GetMoments()
+            assert len(ms) == eq.Q
+            syms = sorted(set().union(*[m.free_symbols for m in ms]), key=str)
+            for k, m in enumerate(ms):
+                vals = [m.subs(dict(zip(syms, v))) for v in xis] if syms else [m] * len(xis)
+                assert [Fraction(int(sp.Rational(v).p), int(sp.Rational(v).q)) for v in vals] \
+                    == list(rows[k]), (name, worth, k)
+                assert rm.GetIndexFromMoment(m) == k
+        rm = RelaxationMatrix(eq)
+        om = sp.symbols('omega_0:%d' % eq.Q)
+        S = rm.GetDiagonalMatrix(list(om))
+        assert S.shape == (eq.Q, eq.Q) and S[3, 3] == om[3] and S[3, 4] == 0
+        ms = rm.GetMoments()
+        S2 = rm.SetOmegasByMoment({(ms[4], ms[4]): 1.2, (ms[4], ms[5]): 0.1})
+        assert S2[4, 4] == 1.2 and S2[4, 5] == 0.1 and sum(abs(x) for x in S2) == 1.3
