@@ -155,9 +155,51 @@ def cpu_baseline(name, budget_s=12.0):
     t0 = time.perf_counter()
     cpuref.meta_run(st, w['es'], configs.SC_G, 1.0, nsteps=k, mrt=mats)
     dt = time.perf_counter() - t0
-    return {'value': V * k / dt / 1e6, 'unit': 'MLUPS', 'cores': 1, 'kind': 'port',
-            'sample': '%s %s %s FP64 (AoS, as CTypes) on %s, %d steps in %.1f s'
-                      % (w['xi'], w['es'], w['coll'].upper(), 'x'.join(map(str, dims)), k, dt)}
+    out = {'value': V * k / dt / 1e6, 'unit': 'MLUPS', 'cores': 1, 'kind': 'port',
+           'sample': '%s %s %s FP64 (AoS, as CTypes) on %s, %d steps in %.1f s'
+                     % (w['xi'], w['es'], w['coll'].upper(), 'x'.join(map(str, dims)), k, dt),
+           'cpu_model': cpu_model()}
+    # the other physics of the workloads, same serial oracle, a few seconds each
+    out['other_samples'] = {
+        'd2q9_e4_128_droplet': _cpu_sample('D2Q9', 'D2E4', (128, 128),
+                                           configs.droplet2d_noise(), 4.0),
+        'd2q9_e8_512_droplets': _cpu_sample('D2Q9', 'D2E8', (512, 512),
+                                            configs.droplets2d(L=512, count=4,
+                                                               seed=configs.SEED0 + 2), 4.0),
+    }
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and its core count."""
+    name = '?'
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                name = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return '%s (%d logical CPUs visible)' % (name, os.cpu_count() or 0)
+
+
+def _cpu_sample(xi, es, dims, n0, budget_s):
+    """MLUPS of the oracle's BGK step (AoS, one core) on a bounded sample."""
+    from idmx import configs
+    from oracle import cpuref
+    V = int(np.prod(dims))
+    st = cpuref.State(xi, dims, aos=True)
+    tmp = cpuref.State(xi, dims)
+    cpuref.init_pop(tmp, n0, np.zeros(V * len(dims)))
+    st.pop[:] = tmp.pop.reshape(st.Q, V).T.ravel()
+    t0 = time.perf_counter()
+    cpuref.meta_run(st, es, configs.SC_G, 1.0, nsteps=1)
+    k = max(1, int(budget_s / max(time.perf_counter() - t0, 1e-6)))
+    t0 = time.perf_counter()
+    cpuref.meta_run(st, es, configs.SC_G, 1.0, nsteps=k)
+    dt = time.perf_counter() - t0
+    return {'mlups': round(V * k / dt / 1e6, 3), 'steps': k, 'seconds': round(dt, 2),
+            'lattice': 'x'.join(map(str, dims)), 'stencil': es}
 
 
 def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
