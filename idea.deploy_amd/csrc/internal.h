@@ -62,6 +62,8 @@ struct SimOps {
   int (*to_f64)(idmx_sc_s*, const void* pop, double* dst);
   int (*from_f64)(idmx_sc_s*, void* pop, const double* src);
   int (*launch_ok)(int tx, int ty, int variant);
+  int (*init_pop_tab)(idmx_sc_s*, void* pop, const double* n, const double* u,
+                      const double* coeffs);
 };
 
 const SimOps* get_ops_d2q9_f64(int estencil, int coll);

@@ -506,6 +506,14 @@ int idmx_sc_init_pop(idmx_sc_t s, void* pop, const double* n, const double* u) {
   return s->ops->init_pop(s, pop, n, u);
 }
 
+int idmx_sc_init_pop_coeffs(idmx_sc_t s, void* pop, const double* n, const double* u,
+                            const double* coeffs) {
+  CHECK_ARG(s && pop && n && u && coeffs, "idmx_sc_init_pop_coeffs: invalid argument");
+  for (int i = 0; i < (s->D == 2 ? 6 : 10) * s->Q; i++)
+    CHECK_ARG(std::isfinite(coeffs[i]), "idmx_sc_init_pop_coeffs: non-finite coefficient");
+  return s->ops->init_pop_tab(s, pop, n, u, coeffs);
+}
+
 int idmx_sc_density_psi(idmx_sc_t s, double* n, double* psi, const void* pop) {
   CHECK_ARG(s && n && psi && pop, "idmx_sc_density_psi: invalid argument");
   return s->ops->density_psi(s, n, psi, pop);

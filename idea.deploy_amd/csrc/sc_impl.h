@@ -267,6 +267,16 @@ struct Impl {
     return IDMX_OK;
   }
 
+  static int init_pop_tab(idmx_sc_s* s, void* pop, const double* n, const double* u,
+                          const double* coeffs) {
+    EqTab<NBasis<S::D>::value * S::Q> tab;
+    for (int i = 0; i < NBasis<S::D>::value * S::Q; i++) tab.c[i] = coeffs[i];
+    hipLaunchKernelGGL((k_init_pop_tab<S, T, SHIFT>), dim3(grid_for(s->geo.V)), dim3(256), 0,
+                       s->ctx->stream, (T*)pop, n, u, s->geo, s->desc.rho0, tab);
+    IDMX_HIP(hipGetLastError());
+    return IDMX_OK;
+  }
+
   static int mass_mom(idmx_sc_s* s, const void* pop, double* mass, double* mom) {
     const unsigned grid = std::min<unsigned>(grid_for(s->geo.V), 1024);
     idmx_ctx_s* c = s->ctx;
@@ -325,7 +335,7 @@ struct Impl {
 
   static const SimOps* ops() {
     static const SimOps o = {run, density_psi, fcs, moments, density_velocity, init_pop,
-                             mass_mom, to_f64, from_f64, launch_ok};
+                             mass_mom, to_f64, from_f64, launch_ok, init_pop_tab};
     return &o;
   }
 };

@@ -1274,6 +1274,38 @@ k_init_pop(T* __restrict__ pop, const double* __restrict__ n,
   }
 }
 
+// f_eq from coefficients on the basis coordinates: f_q = sum_k tab[k Q + q]
+// c_k, c_k the coordinates of basis_coords (no force). The tables of the
+// printed-constant initialisation (K_InitPopulationsMeta with its 15-digit
+// literals, idpy/LBM/LBMKernelsMeta.py:282-390) are passed by value.
+template <int NBQ>
+struct EqTab {
+  double c[NBQ];
+};
+
+template <class S, class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_init_pop_tab(T* __restrict__ pop, const double* __restrict__ n, const double* __restrict__ u,
+               Geom geo, double rho0, EqTab<NBasis<S::D>::value * S::Q> tab) {
+  constexpr int NB = NBasis<S::D>::value;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const double ln = n[g];
+    const double ux = u[g], uy = u[g + geo.V], uz = S::D == 3 ? u[g + 2 * geo.V] : 0.0;
+    const double uu = ux * ux + uy * uy + uz * uz;
+    double c[NB], d[NB];
+    basis_coords<S, double, false>(ln, ln, ux, uy, uz, 0.0, 0.0, 0.0, uu, 0.0, c, d);
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) {
+      double f = tab.c[q] * c[0];
+#pragma unroll
+      for (int k = 1; k < NB; k++) f += tab.c[k * S::Q + q] * c[k];
+      if (SHIFT) f -= S::w[q] * rho0;
+      pop[(size_t)q * geo.V + g] = (T)f;
+    }
+  }
+}
+
 // Device population layout <-> float64 SoA (adds / removes the shift).
 template <class S, class T, bool SHIFT>
 __global__ void __launch_bounds__(256)

@@ -30,7 +30,17 @@ class ShanChenMultiPhase(_ShanChenCommon):
     use_ptrs, fp32_flag, dim_sizes, xi_stencil; plus precision ('fp64' |
     'fp32s'), rho0 and step_mode ('fused' | 'two_pass'). fp32_flag selects
     'fp32s'. The fluctuating-LBM kwargs are not supported.
+
+    constants: 'exact' (default; full-double coefficients, mass conserved to
+    round-off, the arithmetic of the legacy class) or 'ref15' (the 15-digit
+    coefficients this class's generated kernels print, evalf() / str() of the
+    sympy expansion, idpy/LBM/Collisions.py:251-273; with tau = 1 the D2Q9
+    step then loses 8.5e-16 of the mass per step, as the reference does).
+    'ref15' applies to InitPopulations, MainLoop, MainLoopMRT and
+    MainLoopWalls, which then run the dense collision with tables holding
+    the printed coefficients; FP64 only.
     """
+    _PRINTED_CONSTANTS = True
 
     _NEEDED = ['lang', 'cl_kind', 'device', 'custom_types', 'block_size', 'f_stencil',
                'psi_code', 'SC_G', 'tau', 'optimizer_flag', 'e2_val', 'psi_sym',

@@ -18,7 +18,7 @@ from ..IdpyCode import GetParamsClean, GetTenet, HIP_T
 from ..IdpyCode import IdpyMemory
 from ..IdpyCode import IdpySims as IdpySimsIO
 from ..IdpyCode.IdpyMemory import IdpyArrayHIP
-from . import hermite
+from . import hermite, ref15
 from ._engine import SCEngine
 from .stencils import estencil_name, xistencil_name
 
@@ -269,7 +269,9 @@ class _ShanChenCommon(RootLB):
     """What both ShanChenMultiPhase classes share."""
 
     # constructor kwargs of the HIP backend on top of the reference's
-    _EXTRA_PARAMS = ['precision', 'rho0', 'step_mode']
+    _EXTRA_PARAMS = ['precision', 'rho0', 'step_mode', 'constants']
+    # whether the reference class prints its coefficients (constants='ref15')
+    _PRINTED_CONSTANTS = False
 
     def _setup(self, needed, args, kwargs, psi_var):
         if not hasattr(self, 'params_dict'):
@@ -289,6 +291,18 @@ class _ShanChenCommon(RootLB):
         self.params_dict.setdefault('precision',
                                     'fp32s' if self.params_dict.get('fp32_flag') else 'fp64')
         self.params_dict.setdefault('step_mode', 'fused')
+        # 'exact': full-double constants (mass conserved); 'ref15': the 15-digit
+        # coefficients the current class prints into its generated kernels
+        self.params_dict.setdefault('constants', 'exact')
+        if self.params_dict['constants'] not in ('exact', 'ref15'):
+            raise ValueError("constants must be 'exact' or 'ref15'")
+        if self.params_dict['constants'] == 'ref15':
+            if not self._PRINTED_CONSTANTS:
+                raise ValueError("constants='ref15' belongs to the current class "
+                                 "(idmx.LBM.MultiPhase.ShanChenMultiPhase); this class reads "
+                                 "full-double constants")
+            if self.params_dict['precision'] != 'fp64':
+                raise ValueError("constants='ref15' needs FP64 populations")
         self.tenet = GetTenet(self.params_dict)
         self.custom_types = self.params_dict.get('custom_types', LBMTypes)
         if self.params_dict['precision'] == 'fp32s':
@@ -326,11 +340,19 @@ class _ShanChenCommon(RootLB):
 
     def _make_engine(self, mrt=None, forcing='guo', iso_filter=False):
         rho0 = self.params_dict.get('rho0', 1.0)
+        eq_init = None
+        if self.params_dict['constants'] == 'ref15' and forcing == 'guo' and not iso_filter:
+            # the printed coefficients, through the dense collision tables
+            if mrt is None:
+                mrt = ref15.bgk_tables(self.xi_name, self.params_dict['tau'])
+            else:
+                mrt = ref15.mrt_tables_printed(self.xi_name, mrt)
+            eq_init = ref15.init_table(self.xi_name)
         return SCEngine(self.tenet, self.xi_name, self.e_name,
                         tuple(int(s) for s in self.sims_vars['dim_sizes']), self.psi_kind,
                         self.params_dict['SC_G'], self.params_dict['tau'],
                         precision=self.params_dict['precision'], rho0=rho0, mrt=mrt,
-                        forcing=forcing, iso_filter=iso_filter)
+                        forcing=forcing, iso_filter=iso_filter, eq_init=eq_init)
 
     def _scf_engine(self):
         """Engine with the Shan-Chen forcing (BGK), created on first use."""
@@ -532,7 +554,9 @@ class _ShanChenCommon(RootLB):
         key = S.tobytes()
         if key not in self._mrt_engines:
             self._mrt_engines[key] = self._make_engine(
-                mrt=hermite.mrt_kernel_tables(self.xi_name, S))
+                mrt=hermite.mrt_tables(self.xi_name, S)
+                if self.params_dict['constants'] == 'ref15'
+                else hermite.mrt_kernel_tables(self.xi_name, S))
         return self._mrt_engines[key]
 
     def End(self):

@@ -24,7 +24,8 @@ _KEEP = object()  # "leave this setting as it is" for SCEngine.configured
 
 class SCEngine:
     def __init__(self, tenet, xi_name, e_name, dims, psi_kind, G, tau,
-                 precision='fp64', rho0=0.0, mrt=None, forcing='guo', iso_filter=False):
+                 precision='fp64', rho0=0.0, mrt=None, forcing='guo', iso_filter=False,
+                 eq_init=None):
         self.tenet = tenet
         self.xi_name, self.e_name = xi_name, e_name
         self.dims = tuple(int(d) for d in dims)
@@ -35,6 +36,13 @@ class SCEngine:
         self.rho0 = float(rho0) if precision == 'fp32s' else 0.0
         self.pop_dtype = np.float64 if precision == 'fp64' else np.float32
         self.is_mrt = mrt is not None
+        # NB x Q equilibrium coefficients for idmx_sc_init_pop_coeffs (None:
+        # the compiled weights)
+        self.eq_init = None if eq_init is None else \
+            np.ascontiguousarray(eq_init, dtype=np.float64).ravel()
+        if self.eq_init is not None and self.eq_init.size != (6 if self.D == 2 else 10) * self.Q:
+            raise ValueError('eq_init must hold NB x Q = %d values'
+                             % ((6 if self.D == 2 else 10) * self.Q))
         d = _lib.ScDesc()
         d.xi = XI_IDS[xi_name]
         d.estencil = E_IDS[e_name[2:]]
@@ -90,7 +98,11 @@ class SCEngine:
 
     # -- kernels ----------------------------------------------------------
     def init_pop(self, pop, n, u):
-        _lib.call('idmx_sc_init_pop', self.handle, pop.ptr, n.ptr, u.ptr)
+        if self.eq_init is not None:
+            _lib.call('idmx_sc_init_pop_coeffs', self.handle, pop.ptr, n.ptr, u.ptr,
+                      self.eq_init.ctypes.data_as(_lib.c_double_p))
+        else:
+            _lib.call('idmx_sc_init_pop', self.handle, pop.ptr, n.ptr, u.ptr)
 
     def density_psi(self, n, psi, pop):
         _lib.call('idmx_sc_density_psi', self.handle, n.ptr, psi.ptr, pop.ptr)

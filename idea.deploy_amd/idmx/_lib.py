@@ -79,6 +79,8 @@ _SIGS = [
     ('idmx_sc_get_launch', C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ('idmx_sc_init_pop', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ('idmx_sc_init_pop_coeffs', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          c_double_p]),
     ('idmx_sc_density_psi', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ('idmx_sc_force_collide_stream', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]),

@@ -210,6 +210,17 @@ int idmx_sc_pop_bytes(idmx_sc_t sim, size_t* bytes);
 int idmx_sc_init_pop(idmx_sc_t sim, void* pop, const double* n,
                      const double* u);
 
+/* K_InitPopulationsMeta as the current class generates it: f_eq with the
+ * coefficients it prints, 15 significant digits (str(expand(f_eq).evalf()),
+ * idpy/LBM/Equilibria.py:202-210, idpy/LBM/LBMKernelsMeta.py:282-390).
+ * coeffs is a host array of NB x Q doubles (NB = 6 in 2D, 10 in 3D):
+ * f_q = sum_k coeffs[k*Q + q] c_k with the equilibrium coordinates
+ * c_0 = n - 1.5 n u.u, c_a = 3 n u_a, c_aa = 4.5 n u_a^2, c_ab = 9 n u_a u_b
+ * (basis order 2D {1, x, y, xx, yy, xy}, 3D {1, x, y, z, xx, yy, zz, xy,
+ * xz, yz}); the compiled weights give idmx_sc_init_pop. u is SoA float64. */
+int idmx_sc_init_pop_coeffs(idmx_sc_t sim, void* pop, const double* n,
+                            const double* u, const double* coeffs);
+
 /* K_ComputeDensityPsiMeta(n, psi, pop): n = sum_q f_q, psi = psi(n)
  * (idpy/LBM/MultiPhaseKernelsMeta.py:107-167). */
 int idmx_sc_density_psi(idmx_sc_t sim, double* n, double* psi,

@@ -35,6 +35,10 @@
 //     and a body force F[1] += g; MainLoopSimpleSC / MainLoopSimpleWallsSC
 //     (MultiPhase.py:257-349, 467-581) with the Shan-Chen forcing collision
 //     (K_Collision_ShanChenMultiPhase[Walls], MultiPhaseKernels.py:132-304);
+//   * the same MainLoop with the current class's printed 15-digit
+//     coefficients (evalf() / str() of the sympy expansion,
+//     idpy/LBM/Collisions.py:251-273), evaluated as the printed sums
+//     (Terms, cpuref_meta_run_terms / cpuref_init_pop_terms);
 //   * MainLoopIsoFilter (idpy/LBM/MultiPhase.py:1099-1223): the D2Q9 step
 //     with K_IsotropyFilter (idpy/LBM/LBMKernelsMeta.py:895-1080) between
 //     the collision and the push stream;
@@ -541,12 +545,80 @@ void mrt_moment(const MomTabs& t, const MomSparse& sp, const double* f, double n
   }
 }
 
+// Printed sums of the Meta class's generated code (oracle/ref15_terms.py):
+// term t of population q, off[q] <= t < off[q+1], is coef[t] times up to
+// three variables fac[3t..3t+2] (-1 for none) of
+//   v = {ln_0, lu_0_0, lu_0_1, lu_0_2, d_psi_0, d_psi_1, d_psi_2, lpop_q},
+// multiplied left to right; the terms are summed left to right, as a C
+// compiler evaluates `c0*a*b + c1*c - ...` (idpy/LBM/Collisions.py:251-273,
+// idpy/LBM/LBMKernelsMeta.py:282-390).
+struct Terms {
+  const int* off;
+  const double* coef;
+  const int* fac;
+};
+inline double eval_terms(const Terms& t, int q, const double* v) {
+  double acc = 0.;
+  for (int k = t.off[q]; k < t.off[q + 1]; k++) {
+    double p = t.coef[k];
+    for (int j = 0; j < 3; j++)
+      if (t.fac[3 * k + j] >= 0) p = p * v[t.fac[3 * k + j]];
+    acc = k == t.off[q] ? p : acc + p;
+  }
+  return acc;
+}
+
+// The dense collision in the basis-coordinate form the library's tables
+// use: f*_q = sum_k A[q][k] f_k + sum_k beta[k][q] c_k + gamma[k][q] d_k with
+// c, d the coordinates of f_eq and of the raw Guo term on
+// w o {1, xi_a, xi_a xi_b (a <= b)} (2D order {1, x, y, xx, yy, xy}, 3D
+// {1, x, y, z, xx, yy, zz, xy, xz, yz}).
+struct DenseTabs {
+  const double* A;      // Q x Q
+  const double* beta;   // NB x Q
+  const double* gamma;  // NB x Q
+};
+template <class S>
+void dense_collide(const DenseTabs& t, const double* f, double n, const double* u,
+                   const double* F, double* fo) {
+  constexpr int D = S::D, Q = S::Q, NB = D == 2 ? 6 : 10;
+  double uu = 0., Fu = 0.;
+  for (int a = 0; a < D; a++) {
+    uu += u[a] * u[a];
+    Fu += F[a] * u[a];
+  }
+  double c[NB], d[NB];
+  c[0] = n - 1.5 * n * uu;
+  d[0] = -3. * Fu;
+  int k = 1;
+  for (int a = 0; a < D; a++, k++) {
+    c[k] = 3. * n * u[a];
+    d[k] = 3. * F[a];
+  }
+  for (int a = 0; a < D; a++, k++) {
+    c[k] = 4.5 * n * u[a] * u[a];
+    d[k] = 9. * F[a] * u[a];
+  }
+  for (int a = 0; a < D; a++)
+    for (int b = a + 1; b < D; b++, k++) {
+      c[k] = 9. * n * u[a] * u[b];
+      d[k] = 9. * (F[a] * u[b] + F[b] * u[a]);
+    }
+  for (int q = 0; q < Q; q++) {
+    double acc = 0.;
+    for (int j = 0; j < Q; j++) acc += t.A[q * Q + j] * f[j];
+    for (int j = 0; j < NB; j++) acc += t.beta[j * Q + q] * c[j] + t.gamma[j * Q + q] * d[j];
+    fo[q] = acc;
+  }
+}
+
 template <class S>
 void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
                double omega, int psi_kind, const MRTMats* mrt, double*& pop,
                double*& pop_swap, double* n, double* psi, const uint8_t* walls,
                const IsoMats* iso = nullptr, const MomTabs* mom = nullptr,
-               const MomSparse* msp = nullptr) {
+               const MomSparse* msp = nullptr, const Terms* terms = nullptr,
+               const DenseTabs* dense = nullptr) {
   constexpr int D = S::D, Q = S::Q;
   const int64_t V = geo.V;
   // K_ComputeDensityPsiMeta (MultiPhaseKernelsMeta.py:107-167); with walls
@@ -562,7 +634,7 @@ void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
   // 891-1075); with walls K_ForceCollideStreamWallsSCMPMeta (:524-716)
   for (int64_t g = 0; g < V; g++) {
     if (walls && walls[g] != 1) continue;
-    double F[3];
+    double F[3] = {0., 0., 0.};
     sc_force<D>(geo, es, psi, g, G, F);
     const double ln = n[g];
     double lpop[Q];
@@ -581,7 +653,16 @@ void meta_step(const Geo& geo, const EStencil& es, const Ord& ord, double G,
     }
     double feq[Q], fg[Q], fout[Q];
     for (int q = 0; q < Q; q++) eq_guo<S>(q, ln, lu, F, u_dot_u, F_dot_u, &feq[q], &fg[q]);
-    if (mom) {
+    if (dense) {
+      dense_collide<S>(*dense, lpop, ln, lu, F, fout);
+    } else if (terms) {
+      // the printed BGK + Guo sums of K_ForceCollideStreamSCMPMeta
+      double v[8] = {ln, lu[0], lu[1], lu[2], F[0], F[1], F[2], 0.};
+      for (int q = 0; q < Q; q++) {
+        v[7] = lpop[q];
+        fout[q] = eval_terms(*terms, q, v);
+      }
+    } else if (mom) {
       mrt_moment<S>(*mom, *msp, lpop, ln, lu, F, fout);
     } else if (!mrt) {
       // SRT: (1-w) f + w f_eq + (1-w/2) F_guo (Collisions.py:163-185)
@@ -678,6 +759,19 @@ void init_pop(const Geo& geo, const Ord& ord, const double* n, const double* u,
   }
 }
 
+// K_InitPopulationsMeta with its printed sums (LBMKernelsMeta.py:282-390)
+template <class S>
+void init_pop_terms(const Geo& geo, const Ord& ord, const double* n, const double* u,
+                    const Terms& t, double* pop) {
+  constexpr int D = S::D, Q = S::Q;
+  const int64_t V = geo.V;
+  for (int64_t g = 0; g < V; g++) {
+    double v[8] = {n[g], 0., 0., 0., 0., 0., 0., 0.};
+    for (int d = 0; d < D; d++) v[1 + d] = u[g + d * V];
+    for (int q = 0; q < Q; q++) pop[ord(g, q)] = eval_terms(t, q, v);
+  }
+}
+
 bool make_geo(int D, const int* L, Geo* geo) {
   if (D < 2 || D > 3) return false;
   geo->D = D;
@@ -761,6 +855,69 @@ int cpuref_meta_run(int xi_id, const int* L, int QE, const int* E,
     else meta_step<SD3Q19>(geo, es, ord, G, omega, psi_kind, mp, a, b, n, psi, walls);
   }
   if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
+  return 0;
+}
+
+// nsteps of the Meta class MainLoop with the collision given as its printed
+// sums (Terms above; oracle/ref15_terms.py builds them).
+int cpuref_meta_run_terms(int xi_id, const int* L, int QE, const int* E, const double* EW,
+                          double G, int psi_kind, int aos, const int* off, const double* coef,
+                          const int* fac, double* pop, double* pop_swap, double* n,
+                          double* psi, int nsteps) {
+  Geo geo;
+  if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
+  const int Q = xi_id == 0 ? 9 : 19;
+  EStencil es{QE, E, EW};
+  Ord ord{aos != 0, geo.V, Q};
+  Terms t{off, coef, fac};
+  double *a = pop, *b = pop_swap;
+  for (int k = 0; k < nsteps; k++) {
+    if (xi_id == 0)
+      meta_step<SD2Q9>(geo, es, ord, G, 1., psi_kind, nullptr, a, b, n, psi, nullptr, nullptr,
+                       nullptr, nullptr, &t);
+    else
+      meta_step<SD3Q19>(geo, es, ord, G, 1., psi_kind, nullptr, a, b, n, psi, nullptr, nullptr,
+                        nullptr, nullptr, &t);
+  }
+  if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
+  return 0;
+}
+
+// nsteps of the Meta class MainLoop with the dense basis-coordinate tables
+// (DenseTabs above).
+int cpuref_meta_run_dense(int xi_id, const int* L, int QE, const int* E, const double* EW,
+                          double G, int psi_kind, int aos, const double* A, const double* beta,
+                          const double* gamma, double* pop, double* pop_swap, double* n,
+                          double* psi, int nsteps) {
+  Geo geo;
+  if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
+  const int Q = xi_id == 0 ? 9 : 19;
+  EStencil es{QE, E, EW};
+  Ord ord{aos != 0, geo.V, Q};
+  DenseTabs t{A, beta, gamma};
+  double *a = pop, *b = pop_swap;
+  for (int k = 0; k < nsteps; k++) {
+    if (xi_id == 0)
+      meta_step<SD2Q9>(geo, es, ord, G, 1., psi_kind, nullptr, a, b, n, psi, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, &t);
+    else
+      meta_step<SD3Q19>(geo, es, ord, G, 1., psi_kind, nullptr, a, b, n, psi, nullptr, nullptr,
+                        nullptr, nullptr, nullptr, &t);
+  }
+  if (a != pop) memcpy(pop, a, sizeof(double) * geo.V * Q);
+  return 0;
+}
+
+// K_InitPopulationsMeta with printed sums (u in SoA order, u[g + d V]).
+int cpuref_init_pop_terms(int xi_id, const int* L, int aos, const double* n, const double* u,
+                          const int* off, const double* coef, const int* fac, double* pop) {
+  Geo geo;
+  if (!make_geo(xi_id == 0 ? 2 : 3, L, &geo)) return -1;
+  const int Q = xi_id == 0 ? 9 : 19;
+  Ord ord{aos != 0, geo.V, Q};
+  Terms t{off, coef, fac};
+  if (xi_id == 0) init_pop_terms<SD2Q9>(geo, ord, n, u, t, pop);
+  else init_pop_terms<SD3Q19>(geo, ord, n, u, t, pop);
   return 0;
 }
 

@@ -44,6 +44,14 @@ def lib():
                                      C.c_int]
         L.cpuref_iso_filter.argtypes = [C.c_int64, _dp, _dp, _dp]
         L.cpuref_init_pop.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp]
+        L.cpuref_meta_run_terms.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
+                                            C.c_int, C.c_int, _ip, _dp, _ip, _dp, _dp, _dp,
+                                            _dp, C.c_int]
+        L.cpuref_init_pop_terms.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _ip, _dp, _ip,
+                                            _dp]
+        L.cpuref_meta_run_dense.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
+                                            C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                            _dp, C.c_int]
         L.cpuref_init_radial.argtypes = [C.c_int, _ip, C.c_double, C.c_double,
                                          C.c_double, C.c_double, _dp, _dp]
         L.cpuref_init_flat.argtypes = [C.c_int, _ip, C.c_double, C.c_double, C.c_double,
@@ -131,6 +139,43 @@ def init_pop(st, n, u):
     rc = lib().cpuref_init_pop(_xi_id(st.xi), _L(st.dims), int(st.aos),
                                np.ascontiguousarray(n, dtype=np.float64),
                                np.ascontiguousarray(u, dtype=np.float64), st.pop)
+    assert rc == 0
+
+
+def init_pop_ref15(st, n, u):
+    """K_InitPopulationsMeta with the current class's printed 15-digit
+    coefficients (oracle/ref15_terms.py); u is SoA."""
+    from . import ref15_terms
+    off, coef, fac = ref15_terms.equilibrium_terms(st.xi)
+    rc = lib().cpuref_init_pop_terms(_xi_id(st.xi), _L(st.dims), int(st.aos),
+                                     np.ascontiguousarray(n, dtype=np.float64),
+                                     np.ascontiguousarray(u, dtype=np.float64), off, coef,
+                                     fac, st.pop)
+    assert rc == 0
+
+
+def meta_run_ref15(st, estencil, G, tau, psi='exp', nsteps=1):
+    """Meta class MainLoop (idpy/LBM/MultiPhase.py:1015-1097) with the
+    collision evaluated as the printed sums of K_ForceCollideStreamSCMPMeta
+    (15-digit coefficients, oracle/ref15_terms.py)."""
+    from . import ref15_terms
+    QE, Ei, Ew = _estencil(estencil)
+    off, coef, fac = ref15_terms.collision_terms(st.xi, 1.0 / tau)
+    rc = lib().cpuref_meta_run_terms(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, PSI_KIND[psi],
+                                     int(st.aos), off, coef, fac, st.pop, st.pop_swap, st.n,
+                                     st.psi, nsteps)
+    assert rc == 0
+
+
+def meta_run_dense_tables(st, estencil, G, tables, psi='exp', nsteps=1):
+    """Meta class MainLoop(MRT) with the collision as the dense
+    basis-coordinate tables (A Q x Q, beta and gamma NB x Q) that the
+    library's dense kernel takes."""
+    QE, Ei, Ew = _estencil(estencil)
+    A, beta, gamma = (np.ascontiguousarray(t, dtype=np.float64) for t in tables)
+    rc = lib().cpuref_meta_run_dense(_xi_id(st.xi), _L(st.dims), QE, Ei, Ew, G, PSI_KIND[psi],
+                                     int(st.aos), A, beta, gamma, st.pop, st.pop_swap, st.n,
+                                     st.psi, nsteps)
     assert rc == 0
 
 
