@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/idmx.h"
 #include "sc_kernels.h"
@@ -38,6 +39,7 @@ struct idmx_sc_s {
   idmx::Geom geo{};
   size_t pop_elem = 8;       // bytes per population value
   void* mrt_dev = nullptr;   // MRT tables in the working precision
+  idmx::PsiProg* psi_prog_dev = nullptr;  // psi_kind 3: the compiled psi_code
   double last_ms = 0;
   const SimOps* ops = nullptr;
   // fused launch shape (chosen at create time, autotuner may override)
@@ -45,6 +47,9 @@ struct idmx_sc_s {
   int coll = 0;  // 0 BGK, 1 dense MRT, 2 diagonal MRT, 3 BGK with Shan-Chen forcing,
                  // 4 BGK followed by the isotropy filter
   const uint8_t* walls = nullptr;  // device flags, 1 fluid / 0 solid (borrowed)
+  // per-step timing of idmx_sc_run_profiled: 2 events per step + 1, reused
+  std::vector<hipEvent_t> prof_ev;
+  float* prof_ms = nullptr;  // 2 * nsteps host floats while a profiled run is active
   double bforce[3] = {0.0, 0.0, 0.0};  // body force added to the Shan-Chen force
 };
 

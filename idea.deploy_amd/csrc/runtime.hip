@@ -329,7 +329,7 @@ int idmx_sc_create(idmx_ctx_t c, const idmx_sc_desc* d, idmx_sc_t* out) {
   CHECK_ARG(d->xi == IDMX_D2Q9 || d->xi == IDMX_D3Q19, "idmx_sc_create: unknown velocity set");
   CHECK_ARG(d->estencil == IDMX_E4 || d->estencil == IDMX_E6 || d->estencil == IDMX_E8,
             "idmx_sc_create: unknown E-stencil");
-  CHECK_ARG(d->psi_kind >= 0 && d->psi_kind <= 2, "idmx_sc_create: unknown psi kind");
+  CHECK_ARG(d->psi_kind >= 0 && d->psi_kind <= 3, "idmx_sc_create: unknown psi kind");
   CHECK_ARG(d->collision == IDMX_BGK || d->collision == IDMX_MRT ||
                 d->collision == IDMX_BGK_SCF || d->collision == IDMX_BGK_ISO,
             "idmx_sc_create: unknown collision");
@@ -446,6 +446,11 @@ int idmx_sc_destroy(idmx_sc_t s) {
     hipStreamSynchronize(s->ctx->stream);
     hipFree(s->mrt_dev);
   }
+  for (hipEvent_t e : s->prof_ev) hipEventDestroy(e);
+  if (s->psi_prog_dev) {
+    hipStreamSynchronize(s->ctx->stream);
+    hipFree(s->psi_prog_dev);
+  }
   delete s;
   return IDMX_OK;
 }
@@ -473,6 +478,40 @@ int idmx_sc_set_launch(idmx_sc_t s, int tx, int ty, int H, int hist) {
   s->ty = ty;
   s->H = H;
   s->hist = hist;
+  return IDMX_OK;
+}
+
+int idmx_sc_set_psi_program(idmx_sc_t s, const unsigned char* ops, int nops,
+                            const double* consts, int nconsts) {
+  CHECK_ARG(s && ops && nops > 0 && nops <= kPsiOps && nconsts >= 0 && nconsts <= kPsiConsts &&
+                (nconsts == 0 || consts),
+            "idmx_sc_set_psi_program: invalid argument");
+  CHECK_ARG(s->desc.psi_kind == IDMX_PSI_PROGRAM,
+            "idmx_sc_set_psi_program: the simulation was not created with IDMX_PSI_PROGRAM");
+  // validate: known ops, constants used exactly, stack within bounds, one result
+  PsiProg p;
+  p.n = nops;
+  int depth = 0, used = 0;
+  for (int i = 0; i < nops; i++) {
+    const int op = ops[i];
+    CHECK_ARG(op < PSI_NOPS, "idmx_sc_set_psi_program: unknown op");
+    if (op == PSI_N || op == PSI_C) {
+      depth++;
+      if (op == PSI_C) used++;
+    } else if (op == PSI_ADD || op == PSI_SUB || op == PSI_MUL || op == PSI_DIV ||
+               op == PSI_POW || op == PSI_FMAX || op == PSI_FMIN) {
+      depth--;
+    }
+    CHECK_ARG(depth >= 1 && depth <= kPsiStack && used <= nconsts,
+              "idmx_sc_set_psi_program: malformed program (stack or constants)");
+    p.op[i] = (unsigned char)op;
+  }
+  CHECK_ARG(depth == 1 && used == nconsts, "idmx_sc_set_psi_program: malformed program");
+  for (int i = 0; i < nconsts; i++) p.c[i] = consts[i];
+  if (!s->psi_prog_dev) IDMX_HIP(hipMalloc(&s->psi_prog_dev, sizeof(PsiProg)));
+  IDMX_HIP(hipMemcpyAsync(s->psi_prog_dev, &p, sizeof(PsiProg), hipMemcpyHostToDevice,
+                          s->ctx->stream));
+  IDMX_HIP(hipStreamSynchronize(s->ctx->stream));
   return IDMX_OK;
 }
 
@@ -516,6 +555,8 @@ int idmx_sc_init_pop_coeffs(idmx_sc_t s, void* pop, const double* n, const doubl
 
 int idmx_sc_density_psi(idmx_sc_t s, double* n, double* psi, const void* pop) {
   CHECK_ARG(s && n && psi && pop, "idmx_sc_density_psi: invalid argument");
+  CHECK_ARG(s->desc.psi_kind != IDMX_PSI_PROGRAM || s->psi_prog_dev,
+            "idmx_sc_density_psi: IDMX_PSI_PROGRAM without a program");
   return s->ops->density_psi(s, n, psi, pop);
 }
 
@@ -531,6 +572,8 @@ int idmx_sc_run(idmx_sc_t s, void** pop, void** swap, double* n, double* psi, do
   CHECK_ARG(s && pop && swap && *pop && *swap && *pop != *swap,
             "idmx_sc_run: invalid population buffers");
   CHECK_ARG(nsteps >= 0, "idmx_sc_run: nsteps must be >= 0");
+  CHECK_ARG(s->desc.psi_kind != IDMX_PSI_PROGRAM || s->psi_prog_dev,
+            "idmx_sc_run: IDMX_PSI_PROGRAM without a program (idmx_sc_set_psi_program)");
   if (nsteps == 0) {
     s->last_ms = 0;
     return IDMX_OK;
@@ -538,8 +581,20 @@ int idmx_sc_run(idmx_sc_t s, void** pop, void** swap, double* n, double* psi, do
   return s->ops->run(s, pop, swap, n, psi, u, nsteps, flags);
 }
 
+int idmx_sc_run_profiled(idmx_sc_t s, void** pop, void** swap, double* n, double* psi,
+                         double* u, int nsteps, int flags, float* step_ms) {
+  CHECK_ARG(s && step_ms, "idmx_sc_run_profiled: invalid argument");
+  CHECK_ARG(nsteps >= 0 && nsteps <= (1 << 24), "idmx_sc_run_profiled: nsteps out of range");
+  s->prof_ms = step_ms;
+  const int rc = idmx_sc_run(s, pop, swap, n, psi, u, nsteps, flags);
+  s->prof_ms = nullptr;
+  return rc;
+}
+
 int idmx_sc_moments(idmx_sc_t s, const void* pop, double* n, double* psi, double* u) {
   CHECK_ARG(s && pop && n && psi && u, "idmx_sc_moments: invalid argument");
+  CHECK_ARG(s->desc.psi_kind != IDMX_PSI_PROGRAM || s->psi_prog_dev,
+            "idmx_sc_moments: IDMX_PSI_PROGRAM without a program");
   return s->ops->moments(s, pop, n, psi, u);
 }
 

@@ -29,7 +29,20 @@ struct Impl {
     P.mrtA = m;
     P.mrtB = m ? m + S::Q * S::Q : nullptr;
     P.mrtG = m ? m + S::Q * S::Q + NB * S::Q : nullptr;
+    P.prog = s->psi_prog_dev;
     return P;
+  }
+
+  // K_ComputeDensityPsiMeta; a psi_code program has its own instantiation
+  static void launch_density_psi(idmx_sc_s* s, dim3 grid, dim3 block, unsigned shm,
+                                 hipStream_t st, const T* pop, Geom g, Phys<T> P, double* n,
+                                 double* psi, const uint8_t* walls) {
+    if (psi && s->desc.psi_kind == IDMX_PSI_PROGRAM)
+      hipLaunchKernelGGL((k_density_psi<S, T, SHIFT, true>), grid, block, shm, st, pop, g, P, n,
+                         psi, walls);
+    else
+      hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), grid, block, shm, st, pop, g, P, n, psi,
+                         walls);
   }
 
   template <int TX, int TY, bool WF, bool HIST, bool PF = false, bool O3 = false,
@@ -138,11 +151,16 @@ struct Impl {
     }
   }
 
+  // mid: event recorded between the two kernels (per-step profiling), or null
   static hipError_t two_pass(idmx_sc_s* s, const T* pop, T* swap, double* n,
-                             double* psi, double* u, bool wf) {
+                             double* psi, double* u, bool wf, hipEvent_t mid = nullptr) {
     const unsigned grid = grid_for(s->geo.V);
-    hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid), dim3(256), 0,
+    launch_density_psi(s, dim3(grid), dim3(256), 0,
                        s->ctx->stream, pop, s->geo, phys(s), n, psi, s->walls);
+    if (mid) {
+      const hipError_t e = hipEventRecord(mid, s->ctx->stream);
+      if (e != hipSuccess) return e;
+    }
     if (wf)
       hipLaunchKernelGGL((k_fcs<S, ES, T, SHIFT, COLL, true>), dim3(grid), dim3(256), 0,
                          s->ctx->stream, pop, swap, n, psi, s->geo, phys(s), u, s->walls,
@@ -160,10 +178,12 @@ struct Impl {
     // with walls the step is the two-pass one (density and psi of the fluid
     // sites into n / psi, where psi also holds the solid sites' values)
     const bool body = s->bforce[0] != 0.0 || s->bforce[1] != 0.0 || s->bforce[2] != 0.0;
-    const bool twop = (flags & IDMX_RUN_TWO_PASS) != 0 || s->walls || body || COLL >= 3;
+    const bool twop = (flags & IDMX_RUN_TWO_PASS) != 0 || s->walls || body || COLL >= 3 ||
+                      s->desc.psi_kind == IDMX_PSI_PROGRAM;
     if (twop && (!n || !psi)) {
-      set_error(s->walls || body
-                    ? "idmx_sc_run: walls or a body force need the n and psi buffers"
+      set_error(s->walls || body || s->desc.psi_kind == IDMX_PSI_PROGRAM
+                    ? "idmx_sc_run: walls, a body force or a psi program need the n and psi "
+                      "buffers (two-pass step)"
                     : "idmx_sc_run: the two-pass step needs n and psi buffers");
       return IDMX_E_ARG;
     }
@@ -172,6 +192,19 @@ struct Impl {
       return IDMX_E_ARG;
     }
     hipStream_t st = s->ctx->stream;
+    // per-step profiling: events 2k and 2k+1 open step k's first and second
+    // kernel (the fused step has one: its first interval is empty), event
+    // 2 nsteps closes the last step
+    hipEvent_t* pev = nullptr;
+    if (s->prof_ms) {
+      const size_t need = 2 * (size_t)nsteps + 1;
+      while (s->prof_ev.size() < need) {
+        hipEvent_t e;
+        IDMX_HIP(hipEventCreate(&e));
+        s->prof_ev.push_back(e);
+      }
+      pev = s->prof_ev.data();
+    }
     IDMX_HIP(hipEventRecord(s->ctx->ev0, st));
     // pull-form variants: convert the streamed populations to the pull form,
     // step in it, convert back (nsteps + 2 buffer swaps: same parity as
@@ -188,18 +221,25 @@ struct Impl {
       const T* a = (const T*)*pop;
       T* b = (T*)*swap;
       hipError_t e;
+      if (pev) IDMX_HIP(hipEventRecord(pev[2 * k], st));
+      hipEvent_t mid = pev ? pev[2 * k + 1] : nullptr;
       if constexpr (COLL >= 3) {
         // the Shan-Chen forcing and the isotropy filter run on the two-pass
         // kernels only
-        e = two_pass(s, a, b, n, psi, u, wf && last);
+        e = two_pass(s, a, b, n, psi, u, wf && last, mid);
       } else {
-        if (twop) e = two_pass(s, a, b, n, psi, u, wf && last);
-        else if (wf && last) e = fused<true>(s, a, b, n, psi, u);
-        else e = fused<false>(s, a, b, nullptr, nullptr, nullptr);
+        if (twop) {
+          e = two_pass(s, a, b, n, psi, u, wf && last, mid);
+        } else {
+          if (mid) IDMX_HIP(hipEventRecord(mid, st));
+          if (wf && last) e = fused<true>(s, a, b, n, psi, u);
+          else e = fused<false>(s, a, b, nullptr, nullptr, nullptr);
+        }
       }
       if (e != hipSuccess) return hip_fail(e, "idmx_sc_run launch");
       std::swap(*pop, *swap);
     }
+    if (pev && nsteps > 0) IDMX_HIP(hipEventRecord(pev[2 * nsteps], st));
     if constexpr (COLL < 3) if (pull) {
       hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
                          (const T*)*pop, (T*)*swap, s->geo, -1);
@@ -211,11 +251,15 @@ struct Impl {
     float ms = 0;
     IDMX_HIP(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
     s->last_ms = ms;
+    if (pev) {
+      for (int k = 0; k < 2 * nsteps; k++)
+        IDMX_HIP(hipEventElapsedTime(&s->prof_ms[k], pev[k], pev[k + 1]));
+    }
     return IDMX_OK;
   }
 
   static int density_psi(idmx_sc_s* s, double* n, double* psi, const void* pop) {
-    hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid_for(s->geo.V)), dim3(256),
+    launch_density_psi(s, dim3(grid_for(s->geo.V)), dim3(256),
                        0, s->ctx->stream, (const T*)pop, s->geo, phys(s), n, psi, s->walls);
     IDMX_HIP(hipGetLastError());
     return IDMX_OK;
@@ -238,7 +282,7 @@ struct Impl {
 
   static int moments(idmx_sc_s* s, const void* pop, double* n, double* psi, double* u) {
     const unsigned grid = grid_for(s->geo.V);
-    hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid), dim3(256), 0,
+    launch_density_psi(s, dim3(grid), dim3(256), 0,
                        s->ctx->stream, (const T*)pop, s->geo, phys(s), n, psi, s->walls);
     hipLaunchKernelGGL((k_velocity<S, ES, T, SHIFT>), dim3(grid), dim3(256), 0,
                        s->ctx->stream, (const T*)pop, n, psi, s->geo, s->desc.SC_G, u,
@@ -250,7 +294,7 @@ struct Impl {
   // K_ComputeMoments (legacy, LBM.py:1667-1698): n and u = sum xi f / n.
   static int density_velocity(idmx_sc_s* s, const void* pop, double* n, double* u) {
     const unsigned grid = grid_for(s->geo.V);
-    hipLaunchKernelGGL((k_density_psi<S, T, SHIFT>), dim3(grid), dim3(256), 0,
+    launch_density_psi(s, dim3(grid), dim3(256), 0,
                        s->ctx->stream, (const T*)pop, s->geo, phys(s), n, nullptr, nullptr);
     // G = 0: no force term; n stands in for the (unused) psi argument
     hipLaunchKernelGGL((k_velocity<S, ES, T, SHIFT>), dim3(grid), dim3(256), 0,

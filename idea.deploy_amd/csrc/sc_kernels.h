@@ -32,6 +32,22 @@ struct Geom {
 // Number of equilibrium / forcing basis vectors w∘{1, xi_a, xi_a xi_b}.
 template <int D> struct NBasis { static constexpr int value = D == 2 ? 6 : 10; };
 
+// A pseudopotential given as a C expression in n (the psi_code kwarg),
+// compiled on the host into a stack program (idmx/LBM/psi_code.py): PSI_C
+// pushes the next constant, PSI_N the density, the others pop their
+// operands and push the result; at most kPsiStack values are live.
+enum PsiOp : unsigned char {
+  PSI_N = 0, PSI_C, PSI_ADD, PSI_SUB, PSI_MUL, PSI_DIV, PSI_NEG, PSI_EXP, PSI_LOG,
+  PSI_SQRT, PSI_POW, PSI_FABS, PSI_TANH, PSI_SINH, PSI_COSH, PSI_SIN, PSI_COS,
+  PSI_FMAX, PSI_FMIN, PSI_F32, PSI_NOPS
+};
+constexpr int kPsiOps = 48, kPsiConsts = 16, kPsiStack = 8;
+struct PsiProg {
+  int n = 0;                   // number of ops
+  unsigned char op[kPsiOps];
+  double c[kPsiConsts];
+};
+
 template <class T>
 struct Phys {
   T G;       // Shan-Chen coupling SC_G
@@ -41,18 +57,63 @@ struct Phys {
   const T* mrtA;  // Q x Q     M^-1 (1-S) M
   const T* mrtB;  // NB x Q    M^-1 S M v_k
   const T* mrtG;  // NB x Q    M'^-1 (1-S/2) M' v_k
+  const PsiProg* prog;  // psi_kind 3: the program (device copy)
 };
 
 __device__ inline double dexp(double v) { return exp(v); }
 __device__ inline float dexp(float v) { return expf(v); }
 
-// psi(n) for the supported pseudopotentials (psi_code kwarg).
+// psi(n) for the closed-form pseudopotentials (psi_code kwarg).
 template <class T>
 __device__ inline T psi_of(int kind, T n) {
   if (kind == 0) return T(1) - dexp(-n);
   if (kind == 1) return dexp(T(-1) / n);
   return n;
 }
+
+// Evaluates a psi program at n in FP64 (C's double arithmetic; PSI_F32
+// rounds to float as a (float) cast or an f-suffixed literal does). The
+// program and the stack depth are uniform over the wave.
+__device__ inline double psi_prog(const PsiProg* __restrict__ p, double n) {
+  double st[kPsiStack];
+  int sp = 0, ci = 0;
+  const int len = p->n;
+  for (int i = 0; i < len; i++) {
+    const int op = p->op[i];
+    if (op == PSI_N) { st[sp++] = n; continue; }
+    if (op == PSI_C) { st[sp++] = p->c[ci++]; continue; }
+    const double b = st[sp - 1];
+    switch (op) {
+      case PSI_NEG: st[sp - 1] = -b; continue;
+      case PSI_EXP: st[sp - 1] = exp(b); continue;
+      case PSI_LOG: st[sp - 1] = log(b); continue;
+      case PSI_SQRT: st[sp - 1] = sqrt(b); continue;
+      case PSI_FABS: st[sp - 1] = fabs(b); continue;
+      case PSI_TANH: st[sp - 1] = tanh(b); continue;
+      case PSI_SINH: st[sp - 1] = sinh(b); continue;
+      case PSI_COSH: st[sp - 1] = cosh(b); continue;
+      case PSI_SIN: st[sp - 1] = sin(b); continue;
+      case PSI_COS: st[sp - 1] = cos(b); continue;
+      case PSI_F32: st[sp - 1] = (double)(float)b; continue;
+      default: break;
+    }
+    const double a = st[sp - 2];
+    double r;
+    switch (op) {
+      case PSI_ADD: r = a + b; break;
+      case PSI_SUB: r = a - b; break;
+      case PSI_MUL: r = a * b; break;
+      case PSI_DIV: r = a / b; break;
+      case PSI_POW: r = pow(a, b); break;
+      case PSI_FMAX: r = fmax(a, b); break;
+      default: r = fmin(a, b); break;
+    }
+    st[sp - 2] = r;
+    sp--;
+  }
+  return st[0];
+}
+
 
 // Periodic wrap of a coordinate that is at most one period out of range.
 __device__ inline int wrap(int v, int L) {
@@ -1141,7 +1202,9 @@ __device__ inline void push_store_walls(T* __restrict__ dst, const Geom& geo,
 
 // K_ComputeDensityPsiMeta: n, psi from pop (MultiPhaseKernelsMeta.py:107-167;
 // with walls K_ComputeDensityPsiWallsMeta, :42-104: fluid sites only).
-template <class S, class T, bool SHIFT>
+// PROG: psi from the compiled psi_code program (a separate instantiation, so
+// that the closed forms keep their registers and occupancy).
+template <class S, class T, bool SHIFT, bool PROG = false>
 __global__ void __launch_bounds__(256)
 k_density_psi(const T* __restrict__ pop, Geom geo, Phys<T> P,
               double* __restrict__ n, double* __restrict__ psi,
@@ -1154,7 +1217,7 @@ k_density_psi(const T* __restrict__ pop, Geom geo, Phys<T> P,
     for (int q = 0; q < S::Q; q++) s += pop[(size_t)q * geo.V + g];
     const T ln = SHIFT ? P.rho0 + s : s;
     n[g] = (double)ln;
-    if (psi) psi[g] = (double)psi_of(P.psi_kind, ln);
+    if (psi) psi[g] = PROG ? psi_prog(P.prog, (double)ln) : (double)psi_of(P.psi_kind, ln);
   }
 }
 

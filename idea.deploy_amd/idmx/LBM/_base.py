@@ -19,6 +19,7 @@ from ..IdpyCode import IdpyMemory
 from ..IdpyCode import IdpySims as IdpySimsIO
 from ..IdpyCode.IdpyMemory import IdpyArrayHIP
 from . import hermite, ref15
+from .psi_code import compile_psi
 from ._engine import SCEngine
 from .stencils import estencil_name, xistencil_name
 
@@ -166,15 +167,19 @@ def psi_kind_from(psi_sym, psi_code):
             if sp.simplify(expr - c) == 0:
                 return k
     if psi_code is not None:
-        s = re.sub(r'\((PsiType|NType|double|float)\)', '', psi_code)
+        s = re.sub(r'\((PsiType|NType|double)\)', '', psi_code)
         s = re.sub(r'\s+', '', s).replace('ln_0', 'N').replace('ln', 'N')
         s = s.replace('1.0', '1').replace('1.', '1')
         table = {'1-exp(-N)': 'exp', 'exp(-1/N)': 'expinv', 'N': 'linear'}
         if s in table:
             return table[s]
+        # any other C expression: compiled into a stack program that the
+        # kernels evaluate (raises NotImplementedError outside the grammar)
+        compile_psi(psi_code)
+        return 'program'
     raise NotImplementedError(
-        'pseudopotential %r / %r is not one of 1 - exp(-n), exp(-1/n), n'
-        % (psi_sym, psi_code))
+        'pseudopotential %r without psi_code is not one of 1 - exp(-n), exp(-1/n), n'
+        % (psi_sym,))
 
 
 RootLB_Dump_def_name = 'RootLB_dump.hdf5'
@@ -352,7 +357,9 @@ class _ShanChenCommon(RootLB):
                         tuple(int(s) for s in self.sims_vars['dim_sizes']), self.psi_kind,
                         self.params_dict['SC_G'], self.params_dict['tau'],
                         precision=self.params_dict['precision'], rho0=rho0, mrt=mrt,
-                        forcing=forcing, iso_filter=iso_filter, eq_init=eq_init)
+                        forcing=forcing, iso_filter=iso_filter, eq_init=eq_init,
+                        psi_code=self.params_dict['psi_code']
+                        if self.psi_kind == 'program' else None)
 
     def _scf_engine(self):
         """Engine with the Shan-Chen forcing (BGK), created on first use."""
@@ -464,13 +471,23 @@ class _ShanChenCommon(RootLB):
             print('Step:', step)
             nst = step - old_step
             if nst > 0:
+                step_ms = np.zeros(2 * nst, dtype=np.float32) if profiling else None
                 swapped = engine.run(m['pop'], m['pop_swap'], nst, n=m['n'], psi=m['psi'],
-                                     u=fields_u, write_fields=True, two_pass=two_pass)
+                                     u=fields_u, write_fields=True, two_pass=two_pass,
+                                     step_ms=step_ms)
                 if swapped:
                     m['pop'], m['pop_swap'] = m['pop_swap'], m['pop']
-                ms = engine.last_run_ms()
-                prof[step] = {'K_SCStep': np.full(nst, ms / 1e3 / nst),
-                              'device_name': self.tenet.GetDeviceName()}
+                if profiling:
+                    # IdpyLoopProfile's layout: {sequence: {kernel: seconds per
+                    # step, 'device_name': ...}} (idpy/IdpyCode/IdpyCode.py:
+                    # 1913-2085); walls, a body force and the Shan-Chen or
+                    # filtered collisions run the two-pass step
+                    k0, k1 = engine.step_kernel_names(two_pass)
+                    t = step_ms.reshape(nst, 2).astype(np.float64) / 1e3
+                    seq = {k1: t[:, 1], 'device_name': self.tenet.GetDeviceName()}
+                    if k0 is not None:
+                        seq[k0] = t[:, 0]
+                    prof[step] = {0: seq}
             old_step = step
             if len(convergence_functions):
                 checks = [c_f(self) for c_f in convergence_functions]

@@ -12,7 +12,7 @@ from .. import _lib
 
 XI_IDS = {'D2Q9': 0, 'D3Q19': 1}
 E_IDS = {'E4': 4, 'E6': 6, 'E8': 8}
-PSI_KINDS = {'exp': 0, 'expinv': 1, 'linear': 2}
+PSI_KINDS = {'exp': 0, 'expinv': 1, 'linear': 2, 'program': 3}
 PRECISIONS = {'fp64': 0, 'fp32s': 1}
 
 RUN_WRITE_FIELDS = 1
@@ -25,7 +25,7 @@ _KEEP = object()  # "leave this setting as it is" for SCEngine.configured
 class SCEngine:
     def __init__(self, tenet, xi_name, e_name, dims, psi_kind, G, tau,
                  precision='fp64', rho0=0.0, mrt=None, forcing='guo', iso_filter=False,
-                 eq_init=None):
+                 eq_init=None, psi_code=None):
         self.tenet = tenet
         self.xi_name, self.e_name = xi_name, e_name
         self.dims = tuple(int(d) for d in dims)
@@ -82,6 +82,14 @@ class SCEngine:
         nb = C.c_size_t()
         _lib.call('idmx_sc_pop_bytes', h, C.byref(nb))
         self.pop_bytes = nb.value
+        if psi_kind == 'program':
+            if psi_code is None:
+                raise ValueError("psi_kind 'program' needs psi_code")
+            from .psi_code import compile_psi
+            ops, consts = compile_psi(psi_code)
+            _lib.call('idmx_sc_set_psi_program', h, ops.tobytes(), len(ops),
+                      consts.ctypes.data_as(_lib.c_double_p), len(consts))
+            self.psi_program = (ops, consts)
 
     def close(self):
         if self.handle:
@@ -112,16 +120,38 @@ class SCEngine:
                   n.ptr, psi.ptr)
 
     def run(self, pop, pop_swap, nsteps, n=None, psi=None, u=None,
-            write_fields=False, two_pass=False):
+            write_fields=False, two_pass=False, step_ms=None):
         """Advances nsteps; returns True when the state ended in pop_swap's
-        allocation (the caller swaps its references, as M_SwapPop does)."""
+        allocation (the caller swaps its references, as M_SwapPop does).
+        step_ms: a float32 array of 2 * nsteps values to receive each step's
+        kernel times (idmx_sc_run_profiled)."""
         a = C.c_void_p(pop.ptr.value)
         b = C.c_void_p(pop_swap.ptr.value)
         flags = (RUN_WRITE_FIELDS if write_fields else 0) | (RUN_TWO_PASS if two_pass else 0)
         ptr = lambda x: x.ptr if x is not None else None
-        _lib.call('idmx_sc_run', self.handle, C.byref(a), C.byref(b), ptr(n), ptr(psi),
-                  ptr(u), int(nsteps), flags)
+        if step_ms is None:
+            _lib.call('idmx_sc_run', self.handle, C.byref(a), C.byref(b), ptr(n), ptr(psi),
+                      ptr(u), int(nsteps), flags)
+        else:
+            if step_ms.dtype != np.float32 or step_ms.size < 2 * int(nsteps) \
+                    or not step_ms.flags['C_CONTIGUOUS']:
+                raise ValueError('step_ms must be a contiguous float32 array of 2 * nsteps')
+            _lib.call('idmx_sc_run_profiled', self.handle, C.byref(a), C.byref(b), ptr(n),
+                      ptr(psi), ptr(u), int(nsteps), flags,
+                      step_ms.ctypes.data_as(C.POINTER(C.c_float)))
         return a.value != pop.ptr.value
+
+    def step_kernel_names(self, two_pass=False):
+        """Names of the (first, second) kernel of one step for per-step
+        profiles: the two-pass step's density/psi and force-collide-stream
+        kernels, or the single fused kernel."""
+        body = any(v != 0.0 for v in getattr(self, '_body', (0.0, 0.0, 0.0)))
+        if (two_pass or self.forcing == 'sc' or self.iso_filter or body
+                or getattr(self, '_walls', None) is not None):
+            return ('K_ComputeDensityPsiMeta',
+                    'K_ForceCollideStreamSCMP_MRTMeta' if self.is_mrt
+                    else 'K_ForceCollideStreamSCMPMeta')
+        return (None, 'K_SCStep')
 
     def moments(self, pop, n, psi, u):
         _lib.call('idmx_sc_moments', self.handle, pop.ptr, n.ptr, psi.ptr, u.ptr)

@@ -81,11 +81,16 @@ _SIGS = [
     ('idmx_sc_init_pop', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ('idmx_sc_init_pop_coeffs', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           c_double_p]),
+    ('idmx_sc_set_psi_program', C.c_int, [C.c_void_p, C.c_char_p, C.c_int, c_double_p,
+                                          C.c_int]),
     ('idmx_sc_density_psi', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ('idmx_sc_force_collide_stream', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]),
     ('idmx_sc_run', C.c_int, [C.c_void_p, c_void_pp, c_void_pp, C.c_void_p, C.c_void_p,
                               C.c_void_p, C.c_int, C.c_int]),
+    ('idmx_sc_run_profiled', C.c_int, [C.c_void_p, c_void_pp, c_void_pp, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                       C.POINTER(C.c_float)]),
     ('idmx_sc_moments', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     ('idmx_sc_density_velocity', C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,

@@ -52,7 +52,9 @@ enum { IDMX_E4 = 4, IDMX_E6 = 6, IDMX_E8 = 8 };
 enum {
   IDMX_PSI_EXP = 0,    /* 1 - exp(-n) */
   IDMX_PSI_EXPINV = 1, /* exp(-1/n)   */
-  IDMX_PSI_LINEAR = 2  /* n           */
+  IDMX_PSI_LINEAR = 2, /* n           */
+  IDMX_PSI_PROGRAM = 3 /* any psi_code: a stack program set with
+                          idmx_sc_set_psi_program before the first step */
 };
 /* Collisions: BGK and MRT with the Guo forcing; IDMX_BGK_SCF is BGK with
  * the Shan-Chen forcing (equilibrium velocity u + F/(n omega), no Guo term,
@@ -245,6 +247,17 @@ int idmx_sc_force_collide_stream(idmx_sc_t sim, void* pop_swap,
 int idmx_sc_run(idmx_sc_t sim, void** pop, void** pop_swap, double* n,
                 double* psi, double* u, int nsteps, int flags);
 
+/* idmx_sc_run with per-step kernel times, as IdpyLoopProfile returns them
+ * (idpy/IdpyCode/IdpyCode.py:1913-2085; MainLoop(profiling=True),
+ * idpy/LBM/MultiPhase.py:1048-1097): step_ms holds 2 * nsteps floats,
+ * [2k] the density/psi kernel of step k and [2k+1] its force-collide-stream
+ * kernel on the two-pass step, or [2k] = 0 and [2k+1] the fused kernel;
+ * milliseconds from HIP events on the library stream. The pull-form
+ * conversions before and after the steps are not in any step's time. */
+int idmx_sc_run_profiled(idmx_sc_t sim, void** pop, void** pop_swap, double* n,
+                         double* psi, double* u, int nsteps, int flags,
+                         float* step_ms);
+
 /* ComputeMomentsMeta: n, psi and u = sum xi f / n + F/(2n)
  * (idpy/LBM/MultiPhase.py:1565-1662,
  *  K_ComputeVelocityAfterForceSCMPMeta MultiPhaseKernelsMeta.py:1077-1219). */
@@ -283,6 +296,19 @@ int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
  * the autotuner (idmx.tune) overrides it. No reference counterpart: the
  * reference fixes block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
 int idmx_sc_set_launch(idmx_sc_t sim, int tx, int ty, int H, int hist);
+
+/* The pseudopotential of a simulation created with IDMX_PSI_PROGRAM: the
+ * C expression psi_code that the reference compiles into
+ * K_ComputeDensityPsiMeta (idpy/LBM/MultiPhaseKernelsMeta.py:107-167,
+ * MultiPhase.py:2120), compiled on the host into a stack program of nops
+ * op codes (<= 48) and nconsts FP64 constants (<= 16), evaluated in FP64 at
+ * every site: 0 push n, 1 push the next constant, 2 add, 3 sub, 4 mul,
+ * 5 div, 6 negate, 7 exp, 8 log, 9 sqrt, 10 pow, 11 fabs, 12 tanh, 13 sinh,
+ * 14 cosh, 15 sin, 16 cos, 17 fmax, 18 fmin, 19 round to float; at most 8
+ * values on the stack and exactly one at the end (checked, IDMX_E_ARG
+ * otherwise). The program is copied to the device. */
+int idmx_sc_set_psi_program(idmx_sc_t sim, const unsigned char* ops, int nops,
+                            const double* consts, int nconsts);
 
 /* Walls: `walls` is a device array of V unsigned chars (the reference's
  * FlagType), 1 = fluid, 0 = solid; NULL removes them. The simulation borrows

@@ -89,10 +89,16 @@ constexpr double kCM4 = 1. / (kC2 * kC2); // CM4 = 1/c2^2
 
 // Pseudopotential forms accepted through psi_kind (psi_code is a C
 // expression in the reference, MultiPhase.py:2120, LBM.py:1244).
+// kind 3: a psi_code expression compiled by gcc into a function of n, as the
+// CTypes backend compiles it into K_ComputeDensityPsiMeta
+// (MultiPhaseKernelsMeta.py:107-167); set with cpuref_set_psi_fn.
+double (*g_psi_fn)(double) = nullptr;
+
 inline double psi_of(int kind, double n) {
   switch (kind) {
   case 0: return 1. - exp(-n);   // psi = 1 - exp(-n)
   case 1: return exp(-1. / n);   // psi = exp(-1/n)
+  case 3: return g_psi_fn(n);    // psi = psi_code(n)
   default: return n;             // psi = n
   }
 }
@@ -787,6 +793,12 @@ bool make_geo(int D, const int* L, Geo* geo) {
 }  // namespace
 
 extern "C" {
+
+// The function that psi kind 3 calls (double psi(double n)), or NULL.
+int cpuref_set_psi_fn(void* fn) {
+  g_psi_fn = reinterpret_cast<double (*)(double)>(fn);
+  return 0;
+}
 
 // Velocity set ids: 0 = D2Q9, 1 = D3Q19.
 // Runs nsteps of the legacy class MainLoop (SoA). After the call pop holds

@@ -44,6 +44,7 @@ def lib():
                                      C.c_int]
         L.cpuref_iso_filter.argtypes = [C.c_int64, _dp, _dp, _dp]
         L.cpuref_init_pop.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp]
+        L.cpuref_set_psi_fn.argtypes = [C.c_void_p]
         L.cpuref_meta_run_terms.argtypes = [C.c_int, _ip, C.c_int, _ip, _dp, C.c_double,
                                             C.c_int, C.c_int, _ip, _dp, _ip, _dp, _dp, _dp,
                                             _dp, C.c_int]
@@ -64,6 +65,26 @@ def lib():
     return _LIB
 
 
+def compile_psi_code(psi_code, workdir, var='ln_0'):
+    """Compiles the C expression psi_code into double psi(double var) with
+    gcc (-O3, no floating-point contraction, as the CTypes backend builds
+    K_ComputeDensityPsiMeta) and makes it the oracle's psi kind 'code'.
+    Returns the loaded library (keep it alive while the oracle runs)."""
+    import subprocess
+    src = os.path.join(workdir, 'psi_code.c')
+    so = os.path.join(workdir, 'psi_code.so')
+    with open(src, 'w') as fh:
+        fh.write('#include <math.h>\n'
+                 'typedef double NType; typedef double PsiType;\n'
+                 'double psi(double %s) { return %s; }\n' % (var, psi_code))
+    subprocess.run(['gcc', '-std=c99', '-O3', '-ffp-contract=off', '-fPIC', '-shared', '-o', so,
+                    src, '-lm'], check=True)
+    L = C.CDLL(so)
+    fn = C.cast(L.psi, C.c_void_p).value
+    lib().cpuref_set_psi_fn(C.c_void_p(fn))
+    return L
+
+
 def _xi_id(xi):
     return {'D2Q9': 0, 'D3Q19': 1}[xi]
 
@@ -79,7 +100,7 @@ def _L(dims):
     return np.ascontiguousarray(np.array(list(dims) + [1] * (3 - len(dims)), dtype=np.int32))
 
 
-PSI_KIND = {'exp': 0, 'expinv': 1, 'linear': 2}
+PSI_KIND = {'exp': 0, 'expinv': 1, 'linear': 2, 'code': 3}
 
 
 class State:

@@ -229,3 +229,49 @@ def test_mainloopmrt_with_relaxation_matrix(kind):
     print(kind, 'MainLoopMRT rel err vs dense oracle', e_dense, 'vs moment oracle', e_mom,
           flush=True)
     assert e_dense <= 1e-12
+
+
+@pytest.mark.parametrize('step_mode', ['fused', 'two_pass'])
+def test_mainloop_profiling_per_step_kernel_times(step_mode):
+    """MainLoop(profiling=True) returns IdpyLoopProfile's layout: per
+    checkpoint, sequence 0 maps each kernel name to one time per step
+    (seconds, from HIP events around that step's kernels), plus the device
+    name; the fused step is one kernel, the two-pass step two, and the
+    per-step times add up to the checkpoint's own event timing."""
+    from idmx import configs
+    from idmx.IdpyCode import HIP_T
+    from idmx.LBM.MultiPhase import FStencils, ShanChenMultiPhase, XIStencils
+    n = sp.Symbol('n')
+    sim = ShanChenMultiPhase(lang=HIP_T, dim_sizes=(128, 128), xi_stencil=XIStencils['D2Q9'],
+                             f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln_0)',
+                             psi_sym=1 - sp.exp(-n), SC_G=-5.0 / 3.0, tau=1.0,
+                             step_mode=step_mode)
+    sim.SetDensityField(configs.droplet2d_noise())
+    prof = sim.MainLoop([0, 7, 20], profiling=True)
+    total_ms = sim.engine.last_run_ms()
+    f_prof = sim.sims_idpy_memory['pop'].D2H().ravel()
+    sim.End()
+    names = (['K_SCStep'] if step_mode == 'fused'
+             else ['K_ComputeDensityPsiMeta', 'K_ForceCollideStreamSCMPMeta'])
+    assert sorted(prof) == [7, 20]
+    for step, nst in ((7, 7), (20, 13)):
+        seq = prof[step][0]
+        assert isinstance(seq['device_name'], str)
+        assert sorted(k for k in seq if k != 'device_name') == sorted(names)
+        for k in names:
+            t = seq[k]
+            assert t.shape == (nst,) and np.all(t > 0) and np.all(t < 1.0), (k, t)
+    per_step = sum(prof[20][0][k].sum() for k in names) * 1e3
+    print(step_mode, 'sum of step times', per_step, 'ms, checkpoint', total_ms, 'ms',
+          flush=True)
+    assert per_step <= total_ms * 1.0001 + 1e-3
+    assert per_step >= 0.5 * total_ms
+    # the same trajectory as without profiling
+    sim2 = ShanChenMultiPhase(lang=HIP_T, dim_sizes=(128, 128), xi_stencil=XIStencils['D2Q9'],
+                              f_stencil=FStencils['D2E4'], psi_code='1. - exp(-ln_0)',
+                              psi_sym=1 - sp.exp(-n), SC_G=-5.0 / 3.0, tau=1.0,
+                              step_mode=step_mode)
+    sim2.SetDensityField(configs.droplet2d_noise())
+    assert sim2.MainLoop([0, 7, 20]) is None
+    assert np.array_equal(sim2.sims_idpy_memory['pop'].D2H().ravel(), f_prof)
+    sim2.End()
