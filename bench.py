@@ -257,12 +257,12 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     t0 = time.perf_counter()
     starts = warmstart.predict_fields(tenet, net, eval_pts)
     t_inf = time.perf_counter() - t0
-    # learned starts fall back to the cold start when non-finite; the last
-    # run also resets those whose residual grows 10x above its best for 5
-    # checks (the growth rule of the fallback controller, off by default)
-    runs = {'batched_cold_fp64': (None, 'fp64', None), 'batched_cold': (None, 'fp32+fp64', None),
-            'batched_warm_fp64': (starts, 'fp64', None), 'batched_warm': (starts, 'fp32+fp64', None),
-            'batched_warm_fp64_growth_fallback': (starts, 'fp64', 5)}
+    # learned starts run under the default fallback controller (mass screen,
+    # non-finite reset, growth rule: 5 checks above 10x the best residual);
+    # the last run switches the growth rule off for comparison
+    runs = {'batched_cold_fp64': (None, 'fp64', 5), 'batched_cold': (None, 'fp32+fp64', 5),
+            'batched_warm_fp64': (starts, 'fp64', 5), 'batched_warm': (starts, 'fp32+fp64', 5),
+            'batched_warm_fp64_no_growth_rule': (starts, 'fp64', None)}
     fields = {}
     for key, (st, mode, gc) in runs.items():
         res, f = sweep.run_sweep(tenet, eval_pts, start=st, mode=mode, launch=l64,
@@ -273,6 +273,7 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
         out[key] = {'wall_s': wall, 'gpu_ms': res.gpu_ms, 'steps_mean': float(res.steps.mean()),
                     'steps_max': int(res.steps.max()), 'converged': int(res.converged.sum()),
                     'fallbacks': int(res.fallback.sum()),
+                    'fallback_reasons': res.reset_counts(),
                     'history': res.history[::max(1, len(res.history) // 12)] + res.history[-1:]}
         if st is not None:
             out[key]['inference_s'] = t_inf
@@ -371,11 +372,15 @@ def run_precision_study(steps=5000):
     return out
 
 
-def run_sweep_batch_only(n_points, Lmax, mode='fp64', max_steps=400000):
+def run_sweep_batch_only(n_points, Lmax, mode='fp64', max_steps=400000, plain_sample=0):
     """The batched sweep alone at a larger scale (the per-point baseline
     would take hours there): n_points centred droplets, L uniform in
     {64, 96, ..., Lmax}, G in [-5.5, -4.2] (SC_G = G/3), R in [0.1 L, 0.35 L],
-    from the cold start to max |dn| < 1e-10 between checks 200 steps apart."""
+    from the cold start to max |dn| < 1e-10 between checks 200 steps apart,
+    at most max_steps steps per point. Reports the step budget the points
+    needed (quantiles, the largest per lattice size, the points left) and,
+    with plain_sample = K, the plain per-point FP64 runs of K points spread
+    over the sizes, for comparison (steps, time, Laplace pressure)."""
     from idmx import configs, sweep, tune
     from idmx.IdpyCode import Tenet
     tenet = Tenet(0)
@@ -389,16 +394,44 @@ def run_sweep_batch_only(n_points, Lmax, mode='fp64', max_steps=400000):
         kw['launch32'], _ = tune.tune_batch(tenet, pts, 'fp32s', use_cache=False)
         kw['launch32_small'] = tune.tune_batch_small(tenet, pts, 'fp32s')
     t_tune = time.perf_counter() - t0
-    res, _ = sweep.run_sweep(tenet, pts, mode=mode, max_steps=max_steps, **kw,
-                             progress=lambda *h: print('progress', *h, file=sys.stderr,
-                                                       flush=True))
+    res, fields = sweep.run_sweep(tenet, pts, mode=mode, max_steps=max_steps, **kw,
+                                  progress=lambda *h: print('progress', *h, file=sys.stderr,
+                                                            flush=True))
     out = {'n_points': n_points, 'L': [64, Lmax], 'sites': int(sum(p.L * p.L for p in pts)),
            'mode': mode, 'precision_batch': prec, 'tune_s': t_tune, 'launch': l_full,
            'ms_per_step_all_live': tim[','.join(map(str, l_full))],
            'launch_small': l_small, 'wall_s': res.wall_s, 'gpu_ms': res.gpu_ms,
            'steps_mean': float(res.steps.mean()), 'steps_max': int(res.steps.max()),
-           'converged': int(res.converged.sum()),
+           'converged': int(res.converged.sum()), 'max_steps': max_steps,
            'history': res.history[::max(1, len(res.history) // 16)] + res.history[-1:]}
+    ok = res.converged
+    if ok.any():
+        out['steps_quantiles'] = {q: int(np.quantile(res.steps[ok], q / 100))
+                                  for q in (50, 90, 99, 100)}
+    out['steps_max_by_L'] = {int(L): int(max(res.steps[i] for i in range(len(pts))
+                                             if pts[i].L == L and ok[i]) if any(
+                                                 pts[i].L == L and ok[i] for i in range(len(pts)))
+                                         else 0)
+                             for L in sorted({p.L for p in pts})}
+    out['not_converged'] = [{'L': pts[i].L, 'G': round(pts[i].G, 4), 'R': round(pts[i].R, 2),
+                             'steps': int(res.steps[i]), 'finite': bool(np.isfinite(fields[i]).all())}
+                            for i in range(len(pts)) if not ok[i]]
+    if plain_sample:
+        order = sorted(range(len(pts)), key=lambda i: (pts[i].L, i))
+        pick = [order[int(k * (len(order) - 1) / max(1, plain_sample - 1))]
+                for k in range(plain_sample)]
+        sub = [pts[i] for i in pick]
+        st_p, conv_p, wall_p, f_p = sweep.plain_runs(tenet, sub, max_steps=max_steps)
+        rows = []
+        for j, i in enumerate(pick):
+            row = {'L': sub[j].L, 'steps_plain': int(st_p[j]), 'steps_batched': int(res.steps[i]),
+                   'converged_plain': bool(conv_p[j]), 'converged_batched': bool(ok[i])}
+            if conv_p[j] and ok[i] and sweep.is_centred_droplet(f_p[j]) \
+                    and sweep.is_centred_droplet(fields[i]):
+                a, b = sweep.laplace_dp(f_p[j], sub[j].G), sweep.laplace_dp(fields[i], sub[j].G)
+                row['laplace_rel_diff'] = float(abs(a - b) / abs(a))
+            rows.append(row)
+        out['plain_sample'] = {'points': rows, 'wall_s': wall_p}
     tenet.End()
     return out
 
@@ -535,6 +568,10 @@ def main():
     ap.add_argument('--batch-launch', default=None,
                     help='kind,tx,H,grid for --batch-step-study (default: autotuned)')
     ap.add_argument('--sweep-mode', default='fp64', choices=['fp64', 'fp32+fp64'])
+    ap.add_argument('--sweep-max-steps', type=int, default=400000,
+                    help='step budget per point of --sweep-batch-only')
+    ap.add_argument('--sweep-plain-sample', type=int, default=0, metavar='K',
+                    help='with --sweep-batch-only: plain per-point runs of K points')
     ap.add_argument('--sweep-train', type=int, default=512)
     ap.add_argument('--sweep-eval', type=int, default=128)
     ap.add_argument('--sweep-Lmax', type=int, default=256)
@@ -580,7 +617,8 @@ def main():
         print(json.dumps({'batch_step_study': r}), flush=True)
         return
     if args.sweep_batch_only:
-        r = run_sweep_batch_only(args.sweep_batch_only, args.sweep_Lmax, args.sweep_mode)
+        r = run_sweep_batch_only(args.sweep_batch_only, args.sweep_Lmax, args.sweep_mode,
+                                 args.sweep_max_steps, args.sweep_plain_sample)
         print(json.dumps({'sweep_batch_only': r}), flush=True)
         return
     if args.sweep:
