@@ -40,6 +40,7 @@ struct idmx_sc_s {
   size_t pop_elem = 8;       // bytes per population value
   void* mrt_dev = nullptr;   // MRT tables in the working precision
   idmx::PsiProg* psi_prog_dev = nullptr;  // psi_kind 3: the compiled psi_code
+  uint64_t* wmask_dev = nullptr;  // V wall masks of the fused step with walls
   double last_ms = 0;
   const SimOps* ops = nullptr;
   // fused launch shape (chosen at create time, autotuner may override)

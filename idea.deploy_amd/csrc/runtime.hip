@@ -447,9 +447,10 @@ int idmx_sc_destroy(idmx_sc_t s) {
     hipFree(s->mrt_dev);
   }
   for (hipEvent_t e : s->prof_ev) hipEventDestroy(e);
-  if (s->psi_prog_dev) {
+  if (s->psi_prog_dev || s->wmask_dev) {
     hipStreamSynchronize(s->ctx->stream);
-    hipFree(s->psi_prog_dev);
+    if (s->psi_prog_dev) hipFree(s->psi_prog_dev);
+    if (s->wmask_dev) hipFree(s->wmask_dev);
   }
   delete s;
   return IDMX_OK;
@@ -517,6 +518,10 @@ int idmx_sc_set_psi_program(idmx_sc_t s, const unsigned char* ops, int nops,
 
 int idmx_sc_set_walls(idmx_sc_t s, const void* walls) {
   CHECK_ARG(s, "idmx_sc_set_walls: null simulation");
+  // the per-site masks of the fused step with walls (FP64 BGK), allocated
+  // here so that no run allocates
+  if (walls && !s->wmask_dev && s->pop_elem == 8 && s->coll == 0)
+    IDMX_HIP(hipMalloc(&s->wmask_dev, sizeof(uint64_t) * (size_t)s->geo.V));
   s->walls = (const uint8_t*)walls;
   return IDMX_OK;
 }

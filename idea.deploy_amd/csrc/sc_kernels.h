@@ -158,6 +158,14 @@ __device__ inline void basis_coords(T dn, T n, T ux, T uy, T uz, T Fx, T Fy, T F
   }
 }
 
+// Index of the velocity opposite to c_q.
+template <class S>
+__device__ constexpr int opposite(int q) {
+  for (int k = 0; k < S::Q; k++)
+    if (S::cx[k] == -S::cx[q] && S::cy[k] == -S::cy[q] && S::cz[k] == -S::cz[q]) return k;
+  return q;
+}
+
 // Sum of terms with compile-time coefficients, started from its first
 // non-zero term (0 + a is not folded to a: signed zeros), coefficients +-1
 // applied as additions and subtractions. With the loops unrolled every
@@ -910,9 +918,13 @@ k_fused_o3(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<T
 // precomputed once per block (the periodic wrap of the halo included).
 // Copies of 16 bytes per lane (global_load_lds_dwordx4, with rows widened
 // to 16-byte groups) measured 15-20 % slower on this step.
-template <class S, class ES, class T, int TX, int TY, int COLL>
+template <class S, class ES, class T, int TX, int TY, int COLL, bool WALLS = false>
 struct DmaShape {
   static constexpr int D = S::D, Q = S::Q, R = ES::R;
+  // with walls two more planes are staged per lead plane, each laid out as
+  // a population: the psi the caller stored (used at solid sites) and the
+  // per-site wall masks (64-bit)
+  static constexpr int NQ = Q + (WALLS ? 2 : 0);
   static constexpr int NT = TX * TY;
   static constexpr int NW = NT / 64;
   static constexpr int PX = TX + 2 * R;
@@ -927,9 +939,16 @@ struct DmaShape {
   static constexpr int NB = NBasis<D>::value;
   // collision tables copied to LDS: MRT rates (diagonal S) or A, B, G (dense)
   static constexpr int NTAB = COLL == 2 ? Q : COLL == 1 ? Q * Q + 2 * NB * Q : 1;
-  static constexpr int LDS = Q * QB + NS * NP * (int)sizeof(T) + NTAB * (int)sizeof(T);
-  static constexpr bool fits = LDS <= 160 * 1024 && NT % 64 == 0;
+  static constexpr int LDS = NQ * QB + NS * NP * (int)sizeof(T) + NTAB * (int)sizeof(T);
+  static constexpr bool fits =
+      LDS <= 160 * 1024 && NT % 64 == 0 && (!WALLS || sizeof(T) == sizeof(double));
 };
+
+// Wall mask of a site for the fused step with walls: bit q (q >= 1) when the
+// neighbour x + c_q is solid (flag 0: the population pushed there bounces
+// back), bit 63 when the site itself is not fluid (flag != 1: not updated,
+// its psi is the one the caller stored).
+constexpr int kWallSelf = 63;
 
 __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
@@ -953,17 +972,25 @@ __device__ __forceinline__ void dma_dword(uint64_t base, uint32_t voff, uint32_t
       : "memory");
 }
 
-template <class S, class ES, class T, bool SHIFT, int COLL, int TX, int TY>
+// WALLS: the step of MainLoopWalls (fluid sites only, half-way bounce-back
+// at solid sites, the caller's psi at non-fluid sites; K_ForceCollide-
+// StreamWallsSCMPMeta, idpy/LBM/MultiPhaseKernelsMeta.py:524-716) with a
+// body force (bx, by, bz) added to the Shan-Chen force: psi_w is the psi
+// array (its non-fluid sites are staged with the populations), wmask the
+// per-site masks (kWallSelf). FP64 only.
+template <class S, class ES, class T, bool SHIFT, int COLL, int TX, int TY, bool WALLS = false>
 __global__ void __launch_bounds__(TX * TY)
 k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<T> P, int H,
-            int ntx, int nty, int ntiles) {
-  using SH = DmaShape<S, ES, T, TX, TY, COLL>;
+            int ntx, int nty, int ntiles, const double* __restrict__ psi_w = nullptr,
+            const uint64_t* __restrict__ wmask = nullptr, T bx = T(0), T by = T(0),
+            T bz = T(0)) {
+  using SH = DmaShape<S, ES, T, TX, TY, COLL, WALLS>;
   constexpr int D = S::D, Q = S::Q, R = ES::R;
   constexpr int NT = SH::NT, PX = SH::PX, PY = SH::PY, NP = SH::NP, NS = SH::NS;
   constexpr int NK = SH::NK, KW = SH::KW, NW = SH::NW, EPD = SH::EPD, DW = SH::DW;
   static_assert(SH::fits, "staging buffer exceeds the LDS of a CU");
   static_assert(Q <= 63, "vmcnt holds at most 63 outstanding operations");
-  __shared__ __attribute__((aligned(16))) char stage[Q * SH::QB];
+  __shared__ __attribute__((aligned(16))) char stage[SH::NQ * SH::QB];
   __shared__ T ring[NS][PY][PX];
   __shared__ T tab[SH::NTAB];
 
@@ -1035,6 +1062,27 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
           dma_dword(base, soff[kk], stage_lds + (uint32_t)(q * SH::QB + k * 256));
       }
     }
+    if constexpr (WALLS) {
+      // the caller's psi plane and the wall-mask plane, 8 bytes per site
+      // like an FP64 population
+      const uint64_t bp = sgpr_u64((uint64_t)(psi_w + (size_t)mj * (size_t)PS));
+      const uint64_t bm = sgpr_u64((uint64_t)(wmask + (size_t)mj * (size_t)PS));
+#pragma unroll
+      for (int kk = 0; kk < KW; kk++) {
+        const int k = wv + kk * NW;
+        if (k < NK && sval[kk]) {
+          dma_dword(bp, soff[kk], stage_lds + (uint32_t)(Q * SH::QB + k * 256));
+          dma_dword(bm, soff[kk], stage_lds + (uint32_t)((Q + 1) * SH::QB + k * 256));
+        }
+      }
+    }
+  };
+  // staged psi and mask of a haloed site (walls only)
+  auto st_psiw = [&](int site) -> T {
+    return *(const T*)(stage + Q * SH::QB + site * (int)sizeof(T));
+  };
+  auto st_mask = [&](int site) -> uint64_t {
+    return *(const uint64_t*)(stage + (Q + 1) * SH::QB + site * 8);
   };
   auto st_val = [&](int q, int site) -> T {
     return *(const T*)(stage + q * SH::QB + site * (int)sizeof(T));
@@ -1043,6 +1091,7 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
   constexpr int HR = R + 1;
   T fh[HR][Q];
   T dnh[HR];
+  uint64_t mh[HR];  // walls: the own site's mask, with the populations' history
   const int own_site = (D == 3 ? (iy + R) * PX : 0) + ix + R;
 
   // Push targets: byte offsets within a marching plane of the site and its
@@ -1066,8 +1115,27 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
   // that the stores and the copy's loads are in flight together
   T fpend[Q];
   int pend_m = -1;
+  uint64_t pend_mask = 0;
   auto flush = [&]() {
-    if (pend_m >= 0 && own_valid) {
+    if constexpr (WALLS) {
+      // fluid sites only; a population aimed at a solid site goes back to
+      // its own site as the opposite population
+      if (pend_m >= 0 && own_valid && !((pend_mask >> kWallSelf) & 1)) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+          const int cm = D == 3 ? S::cz[q] : S::cy[q];
+          const int mt = cm ? wrap(pend_m + cm, Lm) : pend_m;
+          const int dy = D == 3 ? S::cy[q] : 0;
+          const uint64_t a_push = (uint64_t)(pop_swap) + (uint64_t)q * qbytes +
+                                  (uint64_t)mt * (uint64_t)PS * sizeof(T) +
+                                  toff[dy + 1][S::cx[q] + 1];
+          const uint64_t a_back = (uint64_t)(pop_swap) + (uint64_t)opposite<S>(q) * qbytes +
+                                  (uint64_t)pend_m * (uint64_t)PS * sizeof(T) + toff[1][1];
+          const bool back = q != 0 && ((pend_mask >> q) & 1);
+          __builtin_nontemporal_store(fpend[q], (IDMX_GLOBAL T*)(back ? a_back : a_push));
+        }
+      }
+    } else if (pend_m >= 0 && own_valid) {
 #pragma unroll
       for (int q = 0; q < Q; q++) {
         const int cm = D == 3 ? S::cz[q] : S::cy[q];
@@ -1098,7 +1166,12 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
         s += fh[HR - 1][q];
       }
       dnh[HR - 1] = s;
-      ring[slot][D == 3 ? R + iy : 0][R + ix] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+      T ps = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+      if constexpr (WALLS) {
+        mh[HR - 1] = st_mask(own_site);
+        if ((mh[HR - 1] >> kWallSelf) & 1) ps = st_psiw(own_site);
+      }
+      ring[slot][D == 3 ? R + iy : 0][R + ix] = ps;
     }
     // halo sites: haloed index h of the ring enumerated over the NP - NT
     // sites outside the tile, as in fused_body
@@ -1123,7 +1196,10 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
       T s = 0;
 #pragma unroll
       for (int q = 0; q < Q; q++) s += st_val(q, site);
-      ring[slot][hy][hx] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+      T ps = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+      if constexpr (WALLS)
+        if ((st_mask(site) >> kWallSelf) & 1) ps = st_psiw(site);
+      ring[slot][hy][hx] = ps;
     }
     lds_barrier();
     if (j + 1 < J) issue_copy(j + 1);
@@ -1148,13 +1224,16 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
         }
         const T lpsi = ring[(j - R) % NS][cy][cx];
         T u[3];
-        collide<S, T, SHIFT, COLL>(fh[0], dnh[0], gx, gy, gz, lpsi, PL, fpend, u);
+        collide<S, T, SHIFT, COLL, WALLS>(fh[0], dnh[0], gx, gy, gz, lpsi, PL, fpend, u, bx, by,
+                                          bz);
+        if constexpr (WALLS) pend_mask = mh[0];
       }
       pend_m = m0 + j - 2 * R;
     }
 #pragma unroll
     for (int k = 0; k < HR - 1; k++) {
       dnh[k] = dnh[k + 1];
+      if constexpr (WALLS) mh[k] = mh[k + 1];
 #pragma unroll
       for (int q = 0; q < Q; q++) fh[k][q] = fh[k + 1][q];
     }
@@ -1172,13 +1251,6 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
 // the psi the caller stored there (GetWalls, MultiPhase.py:157-161), which
 // enters the force of their fluid neighbours. walls == nullptr: no walls.
 
-// Index of the velocity opposite to c_q.
-template <class S>
-__device__ constexpr int opposite(int q) {
-  for (int k = 0; k < S::Q; k++)
-    if (S::cx[k] == -S::cx[q] && S::cy[k] == -S::cy[q] && S::cz[k] == -S::cz[q]) return k;
-  return q;
-}
 
 // Push-stream with half-way bounce-back at solid sites.
 template <class S, class T>
@@ -1244,6 +1316,27 @@ __device__ inline void site_xyz(const Geom& geo, int64_t g, int& x, int& y, int&
   const int64_t r = g / geo.L0;
   y = (int)(r % geo.L1);
   z = (int)(r / geo.L1);
+}
+
+// The per-site wall masks of the fused step with walls (kWallSelf), from the
+// flags: recomputed at every run, so flags changed between runs are seen.
+template <class S>
+__global__ void __launch_bounds__(256)
+k_wall_mask(const uint8_t* __restrict__ walls, Geom geo, uint64_t* __restrict__ mask) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < geo.V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    site_xyz(geo, g, x, y, z);
+    uint64_t m = walls[g] != 1 ? (uint64_t(1) << kWallSelf) : uint64_t(0);
+#pragma unroll
+    for (int q = 1; q < S::Q; q++) {
+      const int xt = S::cx[q] ? wrap(x + S::cx[q], geo.L0) : x;
+      const int yt = S::cy[q] ? wrap(y + S::cy[q], geo.L1) : y;
+      const int zt = S::cz[q] ? wrap(z + S::cz[q], geo.L2) : z;
+      if (walls[lex(geo, xt, yt, zt)] == 0) m |= uint64_t(1) << q;
+    }
+    mask[g] = m;
+  }
 }
 
 // K_ForceCollideStreamSCMPMeta / _MRTMeta (MultiPhaseKernelsMeta.py:169-345,

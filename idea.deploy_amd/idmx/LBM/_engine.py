@@ -17,6 +17,7 @@ PRECISIONS = {'fp64': 0, 'fp32s': 1}
 
 RUN_WRITE_FIELDS = 1
 RUN_TWO_PASS = 2
+RUN_FUSED_WALLS = 4
 
 
 _KEEP = object()  # "leave this setting as it is" for SCEngine.configured
@@ -33,6 +34,7 @@ class SCEngine:
         self.Q = 9 if xi_name == 'D2Q9' else 19
         self.V = int(np.prod(self.dims))
         self.precision = precision
+        self.psi_kind = psi_kind
         self.rho0 = float(rho0) if precision == 'fp32s' else 0.0
         self.pop_dtype = np.float64 if precision == 'fp64' else np.float32
         self.is_mrt = mrt is not None
@@ -120,14 +122,15 @@ class SCEngine:
                   n.ptr, psi.ptr)
 
     def run(self, pop, pop_swap, nsteps, n=None, psi=None, u=None,
-            write_fields=False, two_pass=False, step_ms=None):
+            write_fields=False, two_pass=False, step_ms=None, fused_walls=False):
         """Advances nsteps; returns True when the state ended in pop_swap's
         allocation (the caller swaps its references, as M_SwapPop does).
         step_ms: a float32 array of 2 * nsteps values to receive each step's
         kernel times (idmx_sc_run_profiled)."""
         a = C.c_void_p(pop.ptr.value)
         b = C.c_void_p(pop_swap.ptr.value)
-        flags = (RUN_WRITE_FIELDS if write_fields else 0) | (RUN_TWO_PASS if two_pass else 0)
+        flags = (RUN_WRITE_FIELDS if write_fields else 0) | (RUN_TWO_PASS if two_pass else 0) \
+            | (RUN_FUSED_WALLS if fused_walls else 0)
         ptr = lambda x: x.ptr if x is not None else None
         if step_ms is None:
             _lib.call('idmx_sc_run', self.handle, C.byref(a), C.byref(b), ptr(n), ptr(psi),
@@ -146,8 +149,15 @@ class SCEngine:
         profiles: the two-pass step's density/psi and force-collide-stream
         kernels, or the single fused kernel."""
         body = any(v != 0.0 for v in getattr(self, '_body', (0.0, 0.0, 0.0)))
-        if (two_pass or self.forcing == 'sc' or self.iso_filter or body
-                or getattr(self, '_walls', None) is not None):
+        walls = getattr(self, '_walls', None) is not None
+        if (walls and not two_pass and self.precision == 'fp64' and not self.is_mrt
+                and self.e_name.endswith('E4')
+                and self.forcing == 'guo' and not self.iso_filter
+                and self.psi_kind != 'program' and self.V >= (1 << 19)):
+            # the fused step with walls: one kernel per step, the density /
+            # psi kernel only in the run's last (two-pass) step
+            return ('K_ComputeDensityPsiMeta', 'K_SCStep')
+        if (two_pass or self.forcing == 'sc' or self.iso_filter or body or walls):
             return ('K_ComputeDensityPsiMeta',
                     'K_ForceCollideStreamSCMP_MRTMeta' if self.is_mrt
                     else 'K_ForceCollideStreamSCMPMeta')

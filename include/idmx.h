@@ -73,9 +73,14 @@ enum { IDMX_PREC_F64 = 0, IDMX_PREC_F32S = 1 };
 enum {
   IDMX_RUN_WRITE_FIELDS = 1, /* the last step also stores n, psi and the
                                 Guo-shifted u of that step */
-  IDMX_RUN_TWO_PASS = 2      /* use the unfused two-kernel step (density/psi
+  IDMX_RUN_TWO_PASS = 2,     /* use the unfused two-kernel step (density/psi
                                 kernel + force-collide-stream kernel), the
                                 reference's own structure */
+  IDMX_RUN_FUSED_WALLS = 4   /* with walls (FP64 BGK, E4): the fused step with
+                                walls on any lattice size (by default it is
+                                used from 2^19 sites up, the two-pass step
+                                below; the last step of a run that writes the
+                                fields is always the two-pass one) */
 };
 
 typedef struct idmx_ctx_s* idmx_ctx_t;
