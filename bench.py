@@ -170,6 +170,24 @@ def cpu_baseline(name, budget_s=12.0):
     return out
 
 
+def device_clocks():
+    """The GPU's current clock levels as rocm-smi reports them (read only;
+    the same kernel has measured up to 15 % apart on different boxes), or
+    None where rocm-smi is not available."""
+    import subprocess
+    try:
+        p = subprocess.run(['rocm-smi', '--showclocks', '--json'], capture_output=True,
+                           text=True, timeout=20)
+        d = json.loads(p.stdout)
+    except Exception:
+        return None
+    out = {}
+    for card, v in d.items():
+        if isinstance(v, dict):
+            out[card] = {k: val for k, val in v.items() if 'clock' in k.lower() or 'clk' in k.lower()}
+    return out or None
+
+
 def cpu_model():
     """The host CPU's model name (from /proc/cpuinfo) and its core count."""
     name = '?'
@@ -657,6 +675,7 @@ def main():
         'mass_rel_drift': r['mass_rel_drift'],
         'launch': r['launch'],
         'device': r['device'],
+        'device_clocks': device_clocks(),
     }
     extra = {}
     for x in [e for e in args.extra.split(',') if e and e != name]:

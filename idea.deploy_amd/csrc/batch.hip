@@ -16,6 +16,10 @@ hipError_t idmx::batch_step_f64(idmx_batch_s* b, const void* in, void* out) {
   return step_es<double, false>(b, in, out);
 }
 
+hipError_t idmx::batch_two_steps_f64(idmx_batch_s* b, const void* in, void* out) {
+  return two_steps_tx<double, false>(b, in, out);
+}
+
 namespace {
 
 // Streamed <-> pull form of every live domain (kinds 2 and 3).
@@ -33,14 +37,23 @@ hipError_t restream(idmx_batch_s* b, int dir) {
   return hipGetLastError();
 }
 
-// Tile shapes (tx x H) accepted by idmx_batch_set_launch for kind 1.
+// Tile shapes (tx x H) accepted by idmx_batch_set_launch for kinds 1 and 3,
+// and for kind 4 (two steps per pass).
 bool tile_shape_ok(int tx, int H) {
   return (tx == 64 && H == 4) || (tx == 32 && H == 8) || (tx == 16 && H == 16) ||
          (tx == 32 && H == 4);
 }
+bool tile2_shape_ok(int tx, int H) {
+  return (tx == 32 && H == 8) || (tx == 16 && H == 16) || (tx == 32 && H == 16);
+}
 
 hipError_t one_step(idmx_batch_s* b, const void* in, void* out) {
   return b->prec == IDMX_PREC_F64 ? batch_step_f64(b, in, out) : batch_step_f32(b, in, out);
+}
+
+hipError_t two_steps(idmx_batch_s* b, const void* in, void* out) {
+  return b->prec == IDMX_PREC_F64 ? batch_two_steps_f64(b, in, out)
+                                  : batch_two_steps_f32(b, in, out);
 }
 
 template <class T, bool SHIFT>
@@ -181,11 +194,15 @@ int idmx_batch_destroy(idmx_batch_t b) {
 
 int idmx_batch_set_launch(idmx_batch_t b, int kind, int tx, int H, int grid) {
   BCHECK(b, "idmx_batch_set_launch: null batch");
-  BCHECK(kind >= 0 && kind <= 3,
-         "idmx_batch_set_launch: kind is 0 (march), 1 (tile), 2 / 3 (pull-form march / tile)");
+  BCHECK(kind >= 0 && kind <= 4,
+         "idmx_batch_set_launch: kind is 0 (march), 1 (tile), 2 / 3 (pull-form march / tile), "
+         "4 (tile, two steps per pass)");
   if (kind == 0 || kind == 2) {
     BCHECK(tx == 64 || tx == 128 || tx == 256, "idmx_batch_set_launch: march tx is 64, 128 or 256");
     BCHECK(H >= 1, "idmx_batch_set_launch: H >= 1");
+  } else if (kind == 4) {
+    BCHECK(b->estencil == IDMX_E4, "idmx_batch_set_launch: kind 4 needs the E4 force");
+    BCHECK(tile2_shape_ok(tx, H), "idmx_batch_set_launch: kind 4 tiles are 32x8, 16x16, 32x16");
   } else {
     BCHECK(tile_shape_ok(tx, H),
            "idmx_batch_set_launch: tile is 64x4, 32x8, 16x16 or 32x4");
@@ -266,15 +283,20 @@ int idmx_batch_run(idmx_batch_t b, int nsteps) {
   // state in the same buffer) is captured before the timed region, for the
   // buffer parity the steps will see: flipped by the conversion to the
   // pull form.
-  const int G = 2 * std::max(1, std::min(nsteps / 2, 50));
+  // kind 4 advances two steps per launch, so its graph holds an even number
+  // of launches (a multiple of 4 steps) to end in the buffer it started in
+  const bool pairs = b->kind == 4;
+  const int G = pairs ? 4 * std::max(1, std::min(nsteps / 4, 25))
+                      : 2 * std::max(1, std::min(nsteps / 2, 50));
   const int cur_steps = b->cur ^ (pull ? 1 : 0);
   if (b->n_items > 0 && nsteps >= G && (b->graph_steps != G || b->graph_cur != cur_steps)) {
     drop_graph(b);
     hipGraph_t g;
     IDMX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < G; k++) {
+    for (int k = 0; k < (pairs ? G / 2 : G); k++) {
       const int src = (cur_steps + k) & 1;
-      hipError_t e = one_step(b, b->pop[src], b->pop[src ^ 1]);
+      hipError_t e = pairs ? two_steps(b, b->pop[src], b->pop[src ^ 1])
+                           : one_step(b, b->pop[src], b->pop[src ^ 1]);
       if (e != hipSuccess) {
         hipStreamEndCapture(st, &g);
         return hip_fail(e, "idmx_batch_run capture");
@@ -302,10 +324,13 @@ int idmx_batch_run(idmx_batch_t b, int nsteps) {
       IDMX_HIP(hipGraphLaunch(b->graph, st));
       done += b->graph_steps;
     }
-    for (; done < nsteps; done++) {
-      hipError_t e = one_step(b, b->pop[b->cur], b->pop[b->cur ^ 1]);
+    while (done < nsteps) {
+      const bool two = pairs && nsteps - done >= 2;
+      hipError_t e = two ? two_steps(b, b->pop[b->cur], b->pop[b->cur ^ 1])
+                         : one_step(b, b->pop[b->cur], b->pop[b->cur ^ 1]);
       if (e != hipSuccess) return hip_fail(e, "idmx_batch_run launch");
       b->cur ^= 1;
+      done += two ? 2 : 1;
     }
   }
   if (pull) {

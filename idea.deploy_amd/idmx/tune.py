@@ -84,23 +84,26 @@ def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
     return best, timings
 
 
-def batch_shapes():
-    """(kind, tx, H): marching items and tiles, push and pull form."""
+def batch_shapes(estencil='E4'):
+    """(kind, tx, H): marching items and tiles, push and pull form, and with
+    the E4 force tiles advanced two steps per launch (kind 4)."""
     shapes = [(k, tx, H) for k in (0, 2) for tx in (64, 128, 256) for H in (8, 16, 32)]
     shapes += [(k, tx, H) for k in (1, 3) for tx, H in ((64, 4), (32, 8), (16, 16), (32, 4))]
+    if estencil == 'E4':
+        shapes += [(4, tx, H) for tx, H in ((32, 8), (16, 16), (32, 16))]
     return shapes
 
 
-def batch_candidates(cus, items=None):
+def batch_candidates(cus, items=None, estencil='E4'):
     """(kind, tx, H, grid): every shape with three persistent grid sizes,
     plus, where items maps a shape to its work-item count on the batch being
     tuned, a grid of one item per workgroup. One item per workgroup measured
     4 % faster than the best persistent grid for the tiles of a full
     128-domain D2Q9 FP64 batch (3.8 M sites); the grid stays as tuned when
     domains finish, and the surplus workgroups exit at once."""
-    out = [s + (g,) for s in batch_shapes() for g in (cus * 4, cus * 8, cus * 16)]
+    out = [s + (g,) for s in batch_shapes(estencil) for g in (cus * 4, cus * 8, cus * 16)]
     if items:
-        out += [s + (items[s],) for s in batch_shapes()
+        out += [s + (items[s],) for s in batch_shapes(estencil)
                 if items.get(s, 0) > cus * 16]
     return out
 
@@ -111,7 +114,7 @@ def tune_batch(tenet, points, precision, estencil='E4', steps=20, use_cache=True
     fastest plus all timings (ms per step)."""
     from .sweep import Batch, cold_start
     sig = '%d:%d:%d' % (len(points), sum(p.L * p.L for p in points), max(p.L for p in points))
-    key = 'batch3|%s|%s|%s|%s' % (tenet.GetDeviceName(), precision, estencil, sig)
+    key = 'batch4|%s|%s|%s|%s' % (tenet.GetDeviceName(), precision, estencil, sig)
     if use_cache:
         c = _load().get(key)
         if c:
@@ -121,11 +124,11 @@ def tune_batch(tenet, points, precision, estencil='E4', steps=20, use_cache=True
         b.set_density(i, cold_start(p))
     cus = b.get_launch()[3] // 8
     items = {}
-    for s in batch_shapes():
+    for s in batch_shapes(estencil):
         b.set_launch(*s, cus * 4)
         items[s] = b.get_launch()[4]
     timings = {}
-    for cand in batch_candidates(cus, items):
+    for cand in batch_candidates(cus, items, estencil):
         b.set_launch(*cand)
         # each length runs twice and the second run is timed: the first
         # captures the step graph for that length, whose capture and

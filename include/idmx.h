@@ -366,7 +366,11 @@ int idmx_batch_destroy(idmx_batch_t b);
  * LDS (small domains: no dependent row loads). kind 2 / 3: kind 0 / 1 run on
  * the pull form (post-collision values at their own site, read from the
  * source site: aligned stores); idmx_batch_run converts every live domain to
- * that form before its steps and back after them. grid = blocks of a step
+ * that form before its steps and back after them. kind 4 (E4 only): tx x H
+ * tiles (32x8, 16x16, 32x16) advanced two steps per launch, the first
+ * step's populations kept in LDS for the tile plus a 2-site ring (half the
+ * HBM bytes and half the launches per step; an odd step left over runs as
+ * kind 1); the same results as kind 1, bit for bit. grid = blocks of a step
  * launch (0 keeps the current grid); a step launches at most one block per
  * live work item, and each XCD takes a contiguous run of items. Picked by
  * the autotuner (idmx.tune). */
