@@ -256,14 +256,12 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     out['train_data_s'] = time.perf_counter() - t0
     log('training fields: %.1f s, converged %d/%d' % (out['train_data_s'],
                                                        res_tr.converged.sum(), n_train))
-    keep = [i for i in range(n_train) if res_tr.converged[i] and np.isfinite(f_tr[i]).all()]
-    out['train_fields_used'] = len(keep)
-    X = np.stack([warmstart.input_image(train_pts[i].G, train_pts[i].R, train_pts[i].L)[0]
-                  for i in keep])
-    Y = np.stack([warmstart.target_image(f_tr[i], train_pts[i].L)
-                  - warmstart.input_image(train_pts[i].G, train_pts[i].R, train_pts[i].L)[1]
-                  for i in keep]).astype(np.float32)
     t0 = time.perf_counter()
+    conv_fields = [f if c else None for f, c in zip(f_tr, res_tr.converged)]
+    profile = warmstart.ProfileModel()
+    out['profile_fields_used'] = profile.fit(train_pts, conv_fields)
+    X, Y, keep = warmstart.training_set(train_pts, conv_fields, profile)
+    out['train_fields_used'] = len(keep)
     net, hist = warmstart.train(tenet, X, Y, epochs=epochs, batch=32, log=log)
     out['train_s'] = time.perf_counter() - t0
     out['train_loss'] = [float(hist[0]), float(hist[-1])]
@@ -273,28 +271,38 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
                          'steps_max': int(steps_p.max()), 'converged': int(conv_p.sum())}
     log('plain: %.1f s' % wall_p)
     t0 = time.perf_counter()
-    starts = warmstart.predict_fields(tenet, net, eval_pts)
+    starts = warmstart.predict_fields(tenet, net, eval_pts, profile=profile)
     t_inf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    masses = [float(np.sum(sweep.cold_start(p))) for p in eval_pts]
+    prof_starts = profile.predict(eval_pts, masses)
+    t_prof = time.perf_counter() - t0
     # learned starts run under the default fallback controller (mass screen,
-    # non-finite reset, growth rule: 5 checks above 10x the best residual);
-    # the last run switches the growth rule off for comparison
-    runs = {'batched_cold_fp64': (None, 'fp64', 5), 'batched_cold': (None, 'fp32+fp64', 5),
-            'batched_warm_fp64': (starts, 'fp64', 5), 'batched_warm': (starts, 'fp32+fp64', 5),
-            'batched_warm_fp64_no_growth_rule': (starts, 'fp64', None)}
+    # non-finite reset, growth rule: 5 checks above 10x the best residual)
+    # with populations at rest under the Guo forcing; the last three runs
+    # are ablations: the profile model alone (no net), the reference's
+    # f = w n populations, and the growth rule off
+    runs = {'batched_cold_fp64': (None, 'fp64', 5, 'force', 0.0),
+            'batched_cold': (None, 'fp32+fp64', 5, 'force', 0.0),
+            'batched_warm_fp64': (starts, 'fp64', 5, 'force', t_inf),
+            'batched_warm': (starts, 'fp32+fp64', 5, 'force', t_inf),
+            'batched_warm_fp64_profile_only': (prof_starts, 'fp64', 5, 'force', t_prof),
+            'batched_warm_fp64_plain_populations': (starts, 'fp64', 5, 'plain', t_inf),
+            'batched_warm_fp64_no_growth_rule': (starts, 'fp64', None, 'force', t_inf)}
     fields = {}
-    for key, (st, mode, gc) in runs.items():
+    for key, (st, mode, gc, rest, t_start) in runs.items():
         res, f = sweep.run_sweep(tenet, eval_pts, start=st, mode=mode, launch=l64,
                                  launch32=l32, launch_small=s64, launch32_small=s32,
-                                 grow_checks=gc)
+                                 grow_checks=gc, rest_init=rest)
         fields[key] = f
-        wall = res.wall_s + (t_inf if st is not None else 0.0)
+        wall = res.wall_s + t_start
         out[key] = {'wall_s': wall, 'gpu_ms': res.gpu_ms, 'steps_mean': float(res.steps.mean()),
                     'steps_max': int(res.steps.max()), 'converged': int(res.converged.sum()),
                     'fallbacks': int(res.fallback.sum()),
                     'fallback_reasons': res.reset_counts(),
                     'history': res.history[::max(1, len(res.history) // 12)] + res.history[-1:]}
         if st is not None:
-            out[key]['inference_s'] = t_inf
+            out[key]['inference_s'] = t_start
         log('%s: %.1f s, steps mean %.0f, converged %d' % (key, wall, res.steps.mean(),
                                                            res.converged.sum()))
     # initial-field error of warm vs cold start against the converged fields

@@ -110,6 +110,20 @@ int rebuild_items(idmx_batch_s* b) {
     }                         \
   } while (0)
 
+namespace {
+// k_batched_force_rest for one domain in the batch's storage.
+template <class ES>
+void launch_force_rest(idmx_batch_s* b, const BatchDom& D, void* pop, const double* n,
+                       unsigned grid, hipStream_t st) {
+  if (b->prec == IDMX_PREC_F64)
+    hipLaunchKernelGGL((k_batched_force_rest<ES, double, false>), dim3(grid), dim3(256), 0, st,
+                       n, (double*)pop, D.L0, D.L1, D.G, D.psi_kind, 0.0);
+  else
+    hipLaunchKernelGGL((k_batched_force_rest<ES, float, true>), dim3(grid), dim3(256), 0, st,
+                       n, (float*)pop, D.L0, D.L1, D.G, D.psi_kind, D.rho0);
+}
+}  // namespace
+
 extern "C" {
 
 int idmx_batch_create(idmx_ctx_t ctx, const idmx_batch_domain* doms, int ndom, int precision,
@@ -472,6 +486,49 @@ int idmx_batch_symmetrize(idmx_batch_t b, int dom, const double* target_mass) {
     IDMX_HIP(hipGetLastError());
   }
   IDMX_HIP(hipStreamSynchronize(st));
+  return IDMX_OK;
+}
+
+
+int idmx_batch_force_rest(idmx_batch_t b, int dom) {
+  BCHECK(b && dom >= -1 && dom < b->ndom, "idmx_batch_force_rest: invalid argument");
+  hipStream_t st = b->ctx->stream;
+  for (int d = dom < 0 ? 0 : dom; d < (dom < 0 ? b->ndom : dom + 1); d++) {
+    if (dom < 0 && !b->active[d]) continue;
+    const BatchDom& D = b->doms[d];
+    const int64_t V = (int64_t)D.L0 * D.L1;
+    const unsigned grid = (unsigned)std::min<int64_t>((V + 255) / 256, 4096);
+    void* pop = (char*)b->pop[b->cur] + b->elem * D.pop_off;
+    const double* nref = b->n_prev + D.site_off;
+    switch (b->estencil) {
+      case IDMX_E6: launch_force_rest<D2E6>(b, D, pop, nref, grid, st); break;
+      case IDMX_E8: launch_force_rest<D2E8>(b, D, pop, nref, grid, st); break;
+      default: launch_force_rest<D2E4>(b, D, pop, nref, grid, st); break;
+    }
+    IDMX_HIP(hipGetLastError());
+  }
+  IDMX_HIP(hipStreamSynchronize(st));
+  return IDMX_OK;
+}
+
+int idmx_batch_get_populations(idmx_batch_t b, int dom, double* f_host) {
+  BCHECK(b && f_host && dom >= 0 && dom < b->ndom,
+         "idmx_batch_get_populations: invalid argument");
+  const BatchDom& D = b->doms[dom];
+  const int64_t V = (int64_t)D.L0 * D.L1;
+  hipStream_t st = b->ctx->stream;
+  const void* pop = (const char*)b->pop[b->cur] + b->elem * D.pop_off;
+  if (b->prec == IDMX_PREC_F64) {
+    IDMX_HIP(hipMemcpyAsync(f_host, pop, sizeof(double) * 9 * V, hipMemcpyDeviceToHost, st));
+    IDMX_HIP(hipStreamSynchronize(st));
+    return IDMX_OK;
+  }
+  std::vector<float> h(9 * V);
+  IDMX_HIP(hipMemcpyAsync(h.data(), pop, sizeof(float) * 9 * V, hipMemcpyDeviceToHost, st));
+  IDMX_HIP(hipStreamSynchronize(st));
+  for (int q = 0; q < 9; q++)
+    for (int64_t g = 0; g < V; g++)
+      f_host[q * V + g] = (double)h[q * V + g] + D2Q9::w[q] * D.rho0;
   return IDMX_OK;
 }
 

@@ -500,6 +500,46 @@ k_batched_symmetrize(const double* __restrict__ n, T* __restrict__ pop, int L0, 
   }
 }
 
+// Populations of one domain at rest under the Guo forcing, from its density n
+// (FP64, periodic). The collision takes the velocity u = (sum_q xi_q f_q +
+// F/2) / n, so f = w n (zero first moment) puts fluid at rest only where F
+// vanishes: at an interface it starts the fluid at F / (2n), a kick the
+// slowly damped modes of the domain then carry for ~1e4-1e5 steps. Here
+// f = f_eq(n, v) with v = -F / (2n), i.e. first moment -F/2, so the
+// collision sees u = 0 everywhere. F = -G psi(x) sum_k W_k e_k psi(x + e_k)
+// is the domain's Shan-Chen force from n. The result goes to pop (shifted
+// by w rho0 when SHIFT).
+template <class ES, class T, bool SHIFT>
+__global__ void __launch_bounds__(256)
+k_batched_force_rest(const double* __restrict__ n, T* __restrict__ pop, int L0, int L1,
+                     double G, int psi_kind, double rho0) {
+  using S = D2Q9;
+  const int64_t V = (int64_t)L0 * L1;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < V;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int y = (int)(g / L0), x = (int)(g - (int64_t)y * L0);
+    const double nn = n[g];
+    double gx = 0.0, gy = 0.0;
+#pragma unroll
+    for (int k = 0; k < ES::QE; k++) {
+      const int xs = pmod(x + ES::ex[k], L0), ys = pmod(y + ES::ey[k], L1);
+      const double p = psi_of(psi_kind, n[(int64_t)ys * L0 + xs]);
+      gx += ES::w[k] * ES::ex[k] * p;
+      gy += ES::w[k] * ES::ey[k] * p;
+    }
+    const double mg = -G * psi_of(psi_kind, nn);
+    const double vx = -0.5 * mg * gx / nn, vy = -0.5 * mg * gy / nn;
+    const double vv = vx * vx + vy * vy;
+#pragma unroll
+    for (int q = 0; q < S::Q; q++) {
+      const double xv = S::cx[q] * vx + S::cy[q] * vy;
+      double f = S::w[q] * nn * (1.0 + kCM2 * xv + 0.5 * kCM4 * xv * xv - 0.5 * kCM2 * vv);
+      if (SHIFT) f -= S::w[q] * rho0;
+      pop[(size_t)q * V + g] = (T)f;
+    }
+  }
+}
+
 // Conversion of every live domain between streamed populations and the pull
 // form (see k_restream): dir = +1 to the pull form, -1 back. Items tile the
 // live domains, W columns by Hs rows each, 256 threads per item.

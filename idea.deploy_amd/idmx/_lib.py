@@ -112,6 +112,8 @@ _SIGS = [
     ('idmx_batch_check', C.c_int, [C.c_void_p, C.c_void_p]),
     ('idmx_batch_copy_state', C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     ('idmx_batch_symmetrize', C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ('idmx_batch_force_rest', C.c_int, [C.c_void_p, C.c_int]),
+    ('idmx_batch_get_populations', C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ('idmx_batch_last_run_ms', C.c_int, [C.c_void_p, c_double_p]),
     ('idmx_nn_conv3x3_fwd', C.c_int, [C.c_void_p] * 6 + [C.c_int] * 6),
     ('idmx_nn_conv3x3_bwd_data', C.c_int, [C.c_void_p] * 5 + [C.c_int] * 5),

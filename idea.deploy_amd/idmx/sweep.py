@@ -160,6 +160,19 @@ class Batch:
         _lib.call('idmx_batch_symmetrize', self.handle, int(dom),
                   None if m is None else m.ctypes.data)
 
+    def force_rest(self, dom=-1):
+        """Populations of domain dom (-1: every live domain) set to the fluid
+        at rest under the Guo forcing, f = f_eq(n, -F/(2n)), from its
+        check-reference density (idmx_batch_force_rest)."""
+        _lib.call('idmx_batch_force_rest', self.handle, int(dom))
+
+    def get_populations(self, i):
+        """float64 (9, L*L) populations of domain i (shift removed)."""
+        L = self.points[i].L
+        out = np.empty((9, L * L))
+        _lib.call('idmx_batch_get_populations', self.handle, int(i), out.ctypes.data)
+        return out
+
     def copy_state_from(self, other, rest=False):
         """Populations of `other` (converted); rest=True writes the rest
         equilibrium of its density instead (zero momentum)."""
@@ -290,7 +303,7 @@ def screen_starts(points, start, mass_tol=0.05):
 def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
               check_every=200, max_steps=400000, grow_checks=5, estencil='E4',
               launch=None, launch32=None, rho0=1.0, symmetric=True, launch_small=None,
-              launch32_small=None, progress=None, mass_tol=0.05):
+              launch32_small=None, progress=None, mass_tol=0.05, rest_init='force'):
     """Relax every point to steady state; returns SweepResult and the final
     FP64 densities. start: None (cold) or a list of initial densities (None
     entries start cold). mode: 'fp64' (FP64 only) or 'fp32+fp64'.
@@ -303,7 +316,14 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     hand-over are projected onto symmetric fields (a broken symmetry makes
     the droplet drift towards a symmetric lattice position for ~1e5 steps).
     grow_checks / mass_tol: the fallback rules of the module docstring
-    (grow_checks=None switches the growth rule off)."""
+    (grow_checks=None switches the growth rule off).
+    rest_init: populations written from a density, i.e. of a caller-supplied
+    start and at the FP32 -> FP64 hand-over: 'force' (default) the fluid at
+    rest under the Guo forcing, f = f_eq(n, -F/(2n)) (Batch.force_rest);
+    'plain' f = w n, as the cold start and the reference's InitPopulations
+    at u = 0, which start every interface at the velocity F/(2n)."""
+    if rest_init not in ('force', 'plain'):
+        raise ValueError("rest_init is 'force' or 'plain'")
     n = len(points)
     steps = np.zeros(n, dtype=np.int64)
     hist = []
@@ -328,6 +348,8 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
             b32.set_density(i, init[i])
             if warm[i]:
                 _shift_to_mass(b32, i, mass0, symmetric)
+                if rest_init == 'force':
+                    b32.force_rest(i)
 
         def restart32(i):
             n0 = cold_start(points[i])
@@ -349,6 +371,8 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
         # steps), which would move the steady state: restore it
         if symmetric:
             b64.symmetrize(-1, mass=mass0)
+        if rest_init == 'force':
+            b64.force_rest(-1)
         for i in np.nonzero(failed)[0]:   # FP32 blew up: FP64 from the cold start
             restart64(i)
             fb_all[i] = True
@@ -357,6 +381,8 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
             b64.set_density(i, init[i])
             if warm[i]:
                 _shift_to_mass(b64, i, mass0, symmetric)
+                if rest_init == 'force':
+                    b64.force_rest(i)
     else:
         raise ValueError('mode is fp64 or fp32+fp64')
     rem, failed, fb, ms = _relax(b64, np.ones(n, dtype=bool), warm, steps, tol, check_every,

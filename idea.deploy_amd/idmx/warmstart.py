@@ -1,16 +1,31 @@
-"""Learned warm start: a 4-level convolutional U-Net (about 210k parameters)
-that maps a sweep point (G, R, L) plus a coordinate grid to a near-steady
-density field, run and trained by the HIP kernels of libidmx.so (BF16 MFMA
-convolutions with FP32 accumulation, FP32 master weights and Adam).
+"""Learned warm start for the batched droplet sweep.
 
-The net sees an S x S (S = 64) image covering the whole L x L domain. Input
-channels (8, NHWC, bf16): x and y relative to the centre in units of L,
-r / L, the cold-start tanh droplet sampled at the pixel centres, G (scaled),
-R / L, L / 512, and a zero pad channel. It predicts the correction to the
-cold-start image; the warm start is that image resized bilinearly to the
-lattice and shifted by a constant so that its mass equals the cold start's
-(mass is conserved by the dynamics, so the relaxed state is the one the cold
-start reaches).
+Two models, both fitted to equilibrated fields that the batched solver
+produces for other (G, R, L) points:
+
+  ProfileModel  the steady droplet at lattice resolution: liquid and vapour
+                densities, interface width and the angle-averaged profile
+                shape as least-squares polynomials in (G, 1/R); a start is
+                that profile rendered on the lattice with the radius that
+                gives the point its mass.
+  UNet          a 4-level convolutional U-Net (about 210k parameters) run and
+                trained by the HIP kernels of libidmx.so (BF16 MFMA
+                convolutions with FP32 accumulation, FP32 master weights and
+                Adam). It sees an S x S (S = 64) image of the whole L x L
+                domain and predicts the correction to a base image: the
+                profile model's start where that model has a droplet, else
+                the cold-start tanh droplet.
+
+Input channels (8, NHWC, bf16): x and y relative to the centre in units of
+L, r / L, the base image sampled at the pixel centres, G (scaled), R / L,
+L / 512, and a zero pad channel. The warm start is the base field (the
+profile at lattice resolution, or the cold tanh) plus the net's correction
+resized bilinearly to the lattice, shifted by a constant so that its mass
+equals the cold start's (mass is conserved by the dynamics, so the relaxed
+state is the one the cold start reaches). The sweep then writes populations
+at rest under the Guo forcing (idmx_batch_force_rest): with f = w n the
+start would carry the velocity F/(2n) at every interface, and the damped
+modes that kick excites take nearly as many steps as a cold start.
 
 No reference counterpart: idpy has no learned component.
 """
@@ -41,15 +56,17 @@ def n_params():
     return sum(9 * ci * co + co for _, ci, co, _ in ARCH)
 
 
-def input_image(G, R, L, W=2.0):
-    """(S, S, CIN) float32 input channels of one sweep point and the cold
-    start image (S, S) they contain."""
+def input_image(G, R, L, W=2.0, base=None):
+    """(S, S, CIN) float32 input channels of one sweep point and the base
+    image (S, S) they contain: `base` (an S x S image) when given, else the
+    cold-start tanh droplet at the pixel centres."""
     c = (np.arange(S) + 0.5) * L / S - 0.5  # lattice coordinate of pixel centres
     X, Y = np.meshgrid(c, c)
     cx = cy = L // 2
     dx, dy = (X - cx), (Y - cy)
     r = np.sqrt(dx ** 2 + dy ** 2)
-    base = 0.5 * (N_G + N_L) - 0.5 * (N_L - N_G) * np.tanh((r - R) / W)
+    if base is None:
+        base = 0.5 * (N_G + N_L) - 0.5 * (N_L - N_G) * np.tanh((r - R) / W)
     img = np.zeros((S, S, CIN), dtype=np.float32)
     img[..., 0] = dx / L
     img[..., 1] = dy / L
@@ -327,11 +344,39 @@ def train(tenet, inputs, targets, epochs=200, batch=32, lr=2e-3, seed=0, log=Non
     return net, hist
 
 
-def predict_fields(tenet, net, points, cold_mass=None):
-    """Warm-start density fields (float64, L x L) for the sweep points: the
-    network's image (base + correction), resized to the lattice on the device
-    and shifted by a constant to the cold start's mass."""
+def training_set(points, fields, profile=None):
+    """(inputs, targets) for train(): the converged, finite fields among
+    `fields` (None entries are skipped), targets the field's S x S image minus
+    the base image (the profile model's start for that point when it has
+    one, else the cold tanh). Returns (X, Y, indices used)."""
     from .configs import droplet
+    keep = [i for i, f in enumerate(fields) if f is not None and np.isfinite(f).all()]
+    bases = [None] * len(points)
+    if profile is not None:
+        masses = [droplet((points[i].L, points[i].L), points[i].R, 2.0).sum() for i in keep]
+        for i, b in zip(keep, profile.predict([points[i] for i in keep], masses)):
+            bases[i] = b
+    X, Y = [], []
+    for i in keep:
+        p = points[i]
+        bimg = None if bases[i] is None else target_image(bases[i].reshape(p.L, p.L), p.L)
+        x, base = input_image(p.G, p.R, p.L, base=bimg)
+        X.append(x)
+        Y.append(target_image(np.asarray(fields[i]).reshape(p.L, p.L), p.L) - base)
+    return np.stack(X), np.stack(Y).astype(np.float32), keep
+
+
+def predict_fields(tenet, net, points, cold_mass=None, profile=None):
+    """Warm-start density fields (float64, L*L) for the sweep points. With a
+    ProfileModel: its lattice-resolution start plus the net's correction
+    resized to the lattice (where the model has no droplet for a point, the
+    cold tanh image plus the correction, resized); without: the cold image
+    plus the correction. Each field is shifted by a constant to the cold
+    start's mass."""
+    from .configs import droplet
+    masses = [droplet((p.L, p.L), p.R, 2.0).sum() if cold_mass is None else cold_mass[i]
+              for i, p in enumerate(points)]
+    lat = profile.predict(points, masses) if profile is not None else [None] * len(points)
     out = []
     B = net.N
     for i in range(0, len(points), B):
@@ -339,11 +384,15 @@ def predict_fields(tenet, net, points, cold_mass=None):
         xs = np.zeros((B, S, S, CIN), dtype=np.float32)
         bases = []
         for j, p in enumerate(chunk):
-            xs[j], base = input_image(p.G, p.R, p.L)
+            lj = lat[i + j]
+            bimg = None if lj is None else target_image(lj.reshape(p.L, p.L), p.L)
+            xs[j], base = input_image(p.G, p.R, p.L, base=bimg)
             bases.append(base)
         imgs = net.predict_images(xs)
         for j, p in enumerate(chunk):
-            img = np.ascontiguousarray(bases[j] + imgs[j], dtype=np.float32)
+            lj = lat[i + j]
+            img = imgs[j] if lj is not None else bases[j] + imgs[j]
+            img = np.ascontiguousarray(img, dtype=np.float32)
             dimg = _Buf(tenet, img.nbytes)
             dimg.h2d(img)
             dn = _Buf(tenet, p.L * p.L * 8)
@@ -351,7 +400,141 @@ def predict_fields(tenet, net, points, cold_mass=None):
             n = dn.d2h(np.float64, (p.L * p.L,))
             dimg.free()
             dn.free()
-            m_cold = droplet((p.L, p.L), p.R, 2.0).sum() if cold_mass is None else cold_mass[i + j]
-            n = n + (m_cold - n.sum()) / n.size
+            if lj is not None:
+                n = n + lj
+            n = n + (masses[i + j] - n.sum()) / n.size
             out.append(n)
     return out
+
+
+def _centre_dist(L):
+    """Distance of every site of an L x L domain from its centre (L/2, L/2),
+    as the flat (y * L + x) array."""
+    c = L // 2
+    x = np.arange(L) - c
+    return np.sqrt(x[None, :] ** 2 + x[:, None] ** 2).ravel()
+
+
+class ProfileModel:
+    """Lattice-resolution droplet profile learned from equilibrated fields.
+
+    Each training field (a droplet centred on the site (L/2, L/2)) is reduced
+    to its liquid and vapour densities n_l, n_g, the centre R and width W of
+    n = n_g + (n_l - n_g) (1 - tanh((r - R) / W)) / 2 (a linear fit of
+    atanh(1 - 2 phi) against r over the sites with 0.05 < phi < 0.95, phi the
+    normalised density), and the angle average of the remainder, divided by
+    n_l - n_g, in bins of s = (r - R) / W over [-S_MAX, S_MAX]. Least squares
+    then fit n_l, n_g, W and every bin as polynomials in g = (G + 1.6) / 0.25
+    and rho = 10 / R. A start for (G, L, mass) is that profile rendered on
+    the lattice, with R chosen so that the field carries the given mass.
+
+    The model sees the steady state at lattice resolution, which the U-Net's
+    S x S image cannot (one pixel covers L / S sites, several interface
+    widths at L = 256)."""
+
+    S_MAX = 6.0
+    NB = 121
+
+    @staticmethod
+    def _feat(G, R):
+        g, rho = (G + 1.6) / 0.25, 10.0 / R
+        return np.array([1.0, g, g * g, g ** 3, rho, g * rho, rho * rho])
+
+    @classmethod
+    def describe(cls, n, L):
+        """(n_l, n_g, R, W, residual bins) of one field, or None when it is
+        not a droplet centred on the domain's centre site."""
+        n = np.asarray(n, dtype=np.float64).ravel()
+        r = _centre_dist(L)
+        c = L // 2
+        n_l, n_g = n[c * L + c], n[0]
+        if not n_l > n_g + 0.05:
+            return None
+        phi = (n - n_g) / (n_l - n_g)
+        band = (phi > 0.05) & (phi < 0.95)
+        if band.sum() < 8:
+            return None
+        a, b = np.polyfit(r[band], np.arctanh(1 - 2 * phi[band]), 1)
+        if not a > 0:
+            return None
+        W, R = 1.0 / a, -b / a
+        if R < 3 * W or R > L / 2 - 3 * W:
+            return None
+        inner, outer = r < R - 4 * W, r > R + 6 * W
+        if inner.any():
+            n_l = float(n[inner].mean())
+        if outer.any():
+            n_g = float(n[outer].mean())
+        s = (r - R) / W
+        rest = (n - (n_g + (n_l - n_g) * 0.5 * (1 - np.tanh(s)))) / (n_l - n_g)
+        k = np.round((s + cls.S_MAX) / (2 * cls.S_MAX) * (cls.NB - 1)).astype(int)
+        ok = (k >= 0) & (k < cls.NB)
+        cnt = np.bincount(k[ok], minlength=cls.NB)
+        tot = np.bincount(k[ok], rest[ok], minlength=cls.NB)
+        bins = np.where(cnt > 0, tot / np.maximum(cnt, 1), np.nan)
+        return n_l, n_g, R, W, bins
+
+    def fit(self, points, fields):
+        """points: SweepPoint-like (G, L); fields: their equilibrated
+        densities (None or non-droplet entries are skipped). Returns the
+        number of fields used."""
+        rows = []
+        for p, f in zip(points, fields):
+            if f is None or not np.isfinite(f).all():
+                continue
+            d = self.describe(f, p.L)
+            if d is not None:
+                rows.append((p.G,) + d)
+        if len(rows) < 12:
+            raise ValueError('ProfileModel.fit: %d usable fields, need 12' % len(rows))
+        X = np.stack([self._feat(r[0], r[3]) for r in rows])
+        Y = np.array([[r[1], r[2], r[4]] for r in rows])
+        self.coef = np.linalg.lstsq(X, Y, rcond=None)[0]
+        B = np.stack([r[5] for r in rows])
+        self.bin_coef = np.zeros((X.shape[1], self.NB))
+        for j in range(self.NB):
+            m = np.isfinite(B[:, j])
+            if m.sum() >= 2 * X.shape[1]:
+                self.bin_coef[:, j] = np.linalg.lstsq(X[m], B[m, j], rcond=None)[0]
+        self.G_range = (min(r[0] for r in rows), max(r[0] for r in rows))
+        self.n_used = len(rows)
+        return self.n_used
+
+    def render(self, G, L, R):
+        """The learned profile at centre radius R on an L x L lattice (flat)."""
+        f = self._feat(G, R)
+        n_l, n_g, W = f @ self.coef
+        s = (_centre_dist(L) - R) / W
+        grid = np.linspace(-self.S_MAX, self.S_MAX, self.NB)
+        rest = np.interp(s, grid, f @ self.bin_coef, left=0.0, right=0.0)
+        return n_g + (n_l - n_g) * (0.5 * (1 - np.tanh(s)) + rest)
+
+    def predict(self, points, masses):
+        """Start densities of `points` whose totals are `masses`: the radius
+        by Newton steps on the rendered mass, then a uniform shift to the
+        exact mass. None where the mass leaves no droplet of at least three
+        interface widths (the caller starts those cold)."""
+        out = []
+        for p, m in zip(points, masses):
+            R = float(p.R)
+            field = None
+            for _ in range(30):
+                f = self._feat(p.G, R)
+                n_l, n_g, W = f @ self.coef
+                if not (n_l > n_g > 0 and W > 0):
+                    break
+                field = self.render(p.G, p.L, R)
+                dm = m - field.sum()
+                R_new = R + dm / (2 * np.pi * R * (n_l - n_g))
+                if not 3 * W <= R_new <= p.L / 2 - 3 * W:
+                    field = None
+                    break
+                if abs(R_new - R) < 1e-9:
+                    R = R_new
+                    break
+                R = R_new
+            if field is not None:
+                field = self.render(p.G, p.L, R)
+                field = field + (m - field.sum()) / field.size
+            out.append(field)
+        return out

@@ -408,6 +408,20 @@ int idmx_batch_copy_state(idmx_batch_t dst, idmx_batch_t src, int flags);
  * symmetric lattice position over ~1e5 steps), and FP32 storage drifts the
  * mass by ~1e-5 relative over a few thousand steps. */
 int idmx_batch_symmetrize(idmx_batch_t b, int dom, const double* target_mass);
+/* Rewrites the populations of domain dom (-1: every live domain) as the
+ * fluid at rest under the Guo forcing, from its check-reference density n
+ * (the density the last set_density / symmetrize / check left there):
+ * f = f_eq(n, -F / (2n)) with F = -G psi(x) sum_k W_k e_k psi(x + e_k) the
+ * domain's Shan-Chen force, so the collision velocity (sum_q xi_q f_q +
+ * F/2) / n is zero. The reference's InitPopulations at u = 0 (f = w n, as
+ * set_density writes) instead starts each interface at the velocity
+ * F / (2n). For warm starts: a start at the steady density with f = w n
+ * needs ~95 % of a cold start's steps to |dn| < 1e-10, with these
+ * populations a few hundred. No reference counterpart. */
+int idmx_batch_force_rest(idmx_batch_t b, int dom);
+/* Copies domain dom's populations to the host as float64 SoA (9 x V, the
+ * w_q rho0 shift of FP32 storage added back). */
+int idmx_batch_get_populations(idmx_batch_t b, int dom, double* f_host);
 int idmx_batch_last_run_ms(idmx_batch_t b, double* ms);
 
 /* ---- warm-start network kernels ---------------------------------------

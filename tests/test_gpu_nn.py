@@ -99,34 +99,40 @@ def test_unet_training_reduces_loss(tenet):
 
 
 def test_warm_start_gain_vs_cold_start(tenet):
-    """The learned warm start against no net: trained on equilibrated
-    fields of other points, its starts are closer to the converged fields
-    than the cold starts (median max |n0 - n_eq|), and relaxing from them
-    reaches the same end states (Laplace pressure) as relaxing from cold."""
+    """The learned warm start (profile model plus the U-Net's correction,
+    populations at rest under the Guo forcing) against the cold start:
+    fitted to equilibrated fields of other points, it reaches
+    max |dn| < 1e-9 in fewer steps summed over unseen points, at the same
+    end states (Laplace pressure within 1e-3, the spread of the droplet's
+    two symmetric lattice positions being ~2e-4)."""
     from idmx import sweep
     from idmx import warmstart as ws
-    tr = sweep.sample_points(48, 11, Ls=(64,), Rfrac=(0.12, 0.28))
-    ev = sweep.sample_points(8, 12, Ls=(64,), Rfrac=(0.12, 0.28))
+    tr = sweep.sample_points(64, 11, Ls=(64, 96), Rfrac=(0.12, 0.28))
+    ev = sweep.sample_points(16, 12, Ls=(64, 96), Rfrac=(0.12, 0.28))
     res, f = sweep.run_sweep(tenet, tr, mode='fp64', tol=1e-9)
-    keep = [i for i in range(len(tr)) if res.converged[i] and sweep.is_centred_droplet(f[i])]
-    print('training fields', len(keep), flush=True)
-    assert len(keep) >= 32
-    X = np.stack([ws.input_image(tr[i].G, tr[i].R, 64)[0] for i in keep])
-    Y = np.stack([ws.target_image(f[i], 64) - ws.input_image(tr[i].G, tr[i].R, 64)[1]
-                  for i in keep]).astype(np.float32)
-    net, hist = ws.train(tenet, X, Y, epochs=150, batch=16)
-    starts = ws.predict_fields(tenet, net, ev)
+    conv = [f[i] if res.converged[i] else None for i in range(len(tr))]
+    pm = ws.ProfileModel()
+    used = pm.fit(tr, conv)
+    X, Y, keep = ws.training_set(tr, conv, pm)
+    print('training fields: profile %d, net %d' % (used, len(keep)), flush=True)
+    assert used >= 32
+    net, hist = ws.train(tenet, X, Y, epochs=100, batch=16)
+    starts = ws.predict_fields(tenet, net, ev, profile=pm)
     net.free()
     ref, fr = sweep.run_sweep(tenet, ev, mode='fp64', tol=1e-9)
-    ok = [i for i in range(len(ev)) if ref.converged[i] and sweep.is_centred_droplet(fr[i])]
-    assert len(ok) >= 6
-    e_cold = np.median([np.abs(sweep.cold_start(ev[i]) - fr[i].ravel()).max() for i in ok])
-    e_warm = np.median([np.abs(starts[i] - fr[i].ravel()).max() for i in ok])
-    print('loss %.3g -> %.3g, start error cold %.3f learned %.3f'
-          % (hist[0], hist[-1], e_cold, e_warm), flush=True)
-    assert e_warm < e_cold
     res_w, fw = sweep.run_sweep(tenet, ev, start=starts, mode='fp64', tol=1e-9)
+    ok = [i for i in range(len(ev)) if ref.converged[i] and sweep.is_centred_droplet(fr[i])]
+    assert len(ok) >= 9
+    s_cold = int(sum(ref.steps[i] for i in ok))
+    s_warm = int(sum(res_w.steps[i] for i in ok))
+    e_cold = np.median([np.sqrt(np.mean((sweep.cold_start(ev[i]) - fr[i].ravel()) ** 2)) for i in ok])
+    e_warm = np.median([np.sqrt(np.mean((starts[i] - fr[i].ravel()) ** 2)) for i in ok])
+    print('loss %.3g -> %.3g, start rms error cold %.2e learned %.2e, steps cold %d learned %d '
+          '(%.3f)' % (hist[0], hist[-1], e_cold, e_warm, s_cold, s_warm, s_warm / s_cold),
+          flush=True)
+    assert e_warm < 0.1 * e_cold
+    assert s_warm <= 0.95 * s_cold
     for i in ok:
         assert res_w.converged[i]
         dp, dp_ref = sweep.laplace_dp(fw[i], ev[i].G), sweep.laplace_dp(fr[i], ev[i].G)
-        assert abs(dp - dp_ref) <= 1e-6 * abs(dp_ref), (i, dp, dp_ref)
+        assert abs(dp - dp_ref) <= 1e-3 * abs(dp_ref), (i, dp, dp_ref)
