@@ -321,10 +321,18 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     out['plain_end_states_droplet'] = int(sum(drop_p))
     for key in runs:
         both = [a and sweep.is_centred_droplet(fw) for a, fw in zip(drop_p, fields[key])]
-        rel = np.array([abs(dp(fw, p) - dp(fp, p)) / abs(dp(fp, p))
-                        for fw, fp, p, b in zip(fields[key], f_p, eval_pts, both) if b])
+        idx = [i for i, b in enumerate(both) if b]
+        rel = np.array([abs(dp(fields[key][i], eval_pts[i]) - dp(f_p[i], eval_pts[i]))
+                        / abs(dp(f_p[i], eval_pts[i])) for i in idx])
         out[key]['droplet_in_both'] = int(sum(both))
         out[key]['laplace_rel_diff_max'] = float(rel.max()) if rel.size else None
+        if rel.size:
+            w = idx[int(rel.argmax())]
+            q = eval_pts[w]
+            out[key]['laplace_rel_diff_max_point'] = {
+                'index': w, 'G': q.G, 'R': q.R, 'L': q.L, 'plain_steps': int(steps_p[w]),
+                'centre_offset': [float(v) for v in sweep.droplet_centre(fields[key][w])],
+                'plain_centre_offset': [float(v) for v in sweep.droplet_centre(f_p[w])]}
     for key in runs:
         out['speedup_%s_vs_plain' % key] = wall_p / out[key]['wall_s']
     out['speedup_warm_vs_plain'] = max(out['speedup_batched_warm_vs_plain'],

@@ -493,3 +493,18 @@ def plain_runs(tenet, points, tol=1e-10, check_every=200, max_steps=400000, star
             x.Free()
         eng.close()
     return steps, conv, _t.perf_counter() - t0, fields
+
+
+def droplet_centre(n):
+    """Offset (x, y) of a droplet's centre from the domain centre (L/2, L/2):
+    the circular mean of the excess density over each axis (periodic)."""
+    n = np.asarray(n, dtype=np.float64)
+    L = int(round(np.sqrt(n.size)))
+    n = n.reshape(L, L)
+    w = n - n.min()
+    ang = np.exp(2j * np.pi * np.arange(L) / L)
+    out = []
+    for prof in (w.sum(axis=0), w.sum(axis=1)):
+        a = (np.angle((prof * ang).sum()) % (2 * np.pi)) * L / (2 * np.pi)
+        out.append((a - L // 2 + L / 2) % L - L / 2)
+    return tuple(out)
