@@ -251,8 +251,10 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     t0 = time.perf_counter()
     lt, _ = tune.tune_batch(tenet, train_pts, 'fp64', use_cache=False)
     lts = tune.tune_batch_small(tenet, train_pts, 'fp64')
-    res_tr, f_tr = sweep.run_sweep(tenet, train_pts, mode='fp64', tol=1e-8, launch=lt,
-                                   launch_small=lts)
+    res_tr, f_tr = sweep.run_sweep(
+        tenet, train_pts, mode='fp64', tol=1e-8, launch=lt, launch_small=lts,
+        progress=lambda lab, st, live, dn: log('training sweep: %d steps, %d live, max |dn| %.1e'
+                                               % (st, live, dn)))
     out['train_data_s'] = time.perf_counter() - t0
     log('training fields: %.1f s, converged %d/%d' % (out['train_data_s'],
                                                        res_tr.converged.sum(), n_train))
@@ -266,7 +268,8 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     out['train_s'] = time.perf_counter() - t0
     out['train_loss'] = [float(hist[0]), float(hist[-1])]
     # plain per-point FP64 runs from the cold start
-    steps_p, conv_p, wall_p, f_p = sweep.plain_runs(tenet, eval_pts)
+    steps_p, conv_p, wall_p, f_p = sweep.plain_runs(
+        tenet, eval_pts, progress=lambda i, w: log('plain runs: %d points, %.0f s' % (i, w)))
     out['plain_fp64'] = {'wall_s': wall_p, 'steps_mean': float(steps_p.mean()),
                          'steps_max': int(steps_p.max()), 'converged': int(conv_p.sum())}
     log('plain: %.1f s' % wall_p)

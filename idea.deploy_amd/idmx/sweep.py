@@ -451,12 +451,14 @@ def sample_points(count, seed, Ls=tuple(range(64, 513, 32)), G_lit=(-5.5, -4.2),
     return out
 
 
-def plain_runs(tenet, points, tol=1e-10, check_every=200, max_steps=400000, starts=None):
+def plain_runs(tenet, points, tol=1e-10, check_every=200, max_steps=400000, starts=None,
+               progress=None):
     """The unbatched baseline: one FP64 simulation per point, run the way a
     notebook runs MainLoop with a convergence callback, i.e. checkpoints of
     `check_every` steps and a device max |n - n_prev| check after each. A
     run whose residual turns non-finite stops there (as the batched sweep
     drops such a domain), so both spend the same steps on diverging points.
+    progress(points done, seconds) is called after every 32 points.
     Returns (steps, converged, wall seconds, final densities)."""
     import time as _t
     from .IdpyCode import IdpyMemory
@@ -492,6 +494,8 @@ def plain_runs(tenet, points, tol=1e-10, check_every=200, max_steps=400000, star
         for x in (n, nprev, psi, u, a, b):
             x.Free()
         eng.close()
+        if progress is not None and (i + 1) % 32 == 0:
+            progress(i + 1, _t.perf_counter() - t0)
     return steps, conv, _t.perf_counter() - t0, fields
 
 
