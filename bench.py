@@ -409,7 +409,8 @@ def run_precision_study(steps=5000):
     return out
 
 
-def run_sweep_batch_only(n_points, Lmax, mode='fp64', max_steps=400000, plain_sample=0):
+def run_sweep_batch_only(n_points, Lmax, mode='fp64', max_steps=400000, plain_sample=0,
+                         learned=0, epochs=120):
     """The batched sweep alone at a larger scale (the per-point baseline
     would take hours there): n_points centred droplets, L uniform in
     {64, 96, ..., Lmax}, G in [-5.5, -4.2] (SC_G = G/3), R in [0.1 L, 0.35 L],
@@ -417,8 +418,12 @@ def run_sweep_batch_only(n_points, Lmax, mode='fp64', max_steps=400000, plain_sa
     at most max_steps steps per point. Reports the step budget the points
     needed (quantiles, the largest per lattice size, the points left) and,
     with plain_sample = K, the plain per-point FP64 runs of K points spread
-    over the sizes, for comparison (steps, time, Laplace pressure)."""
-    from idmx import configs, sweep, tune
+    over the sizes, for comparison (steps, time, Laplace pressure).
+    learned = M > 0: the points start from the learned warm start, its
+    profile model and net fitted on M other points of the same sizes that
+    the batched solver relaxes to 1e-8 first (at most 300,000 steps each;
+    not counted in the wall time, the prediction is)."""
+    from idmx import configs, sweep, tune, warmstart
     from idmx.IdpyCode import Tenet
     tenet = Tenet(0)
     pts = sweep.sample_points(n_points, configs.SEED0 + 6, Ls=tuple(range(64, Lmax + 1, 32)))
@@ -431,13 +436,44 @@ def run_sweep_batch_only(n_points, Lmax, mode='fp64', max_steps=400000, plain_sa
         kw['launch32'], _ = tune.tune_batch(tenet, pts, 'fp32s', use_cache=False)
         kw['launch32_small'] = tune.tune_batch_small(tenet, pts, 'fp32s')
     t_tune = time.perf_counter() - t0
-    res, fields = sweep.run_sweep(tenet, pts, mode=mode, max_steps=max_steps, **kw,
-                                  progress=lambda *h: print('progress', *h, file=sys.stderr,
-                                                            flush=True))
+    log = lambda *h: print('progress', *h, file=sys.stderr, flush=True)
+    starts, t_inf, info = None, 0.0, None
+    if learned:
+        t0 = time.perf_counter()
+        tr = sweep.sample_points(learned, configs.SEED0 + 7, Ls=tuple(range(64, Lmax + 1, 32)))
+        res_tr, f_tr = sweep.run_sweep(tenet, tr, mode='fp64', tol=1e-8, max_steps=300000,
+                                       launch=l_full, launch_small=l_small, progress=log)
+        t_data = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        conv = [f if c else None for f, c in zip(f_tr, res_tr.converged)]
+        del f_tr
+        profile = warmstart.ProfileModel()
+        n_prof = profile.fit(tr, conv)
+        X, Y, keep = warmstart.training_set(tr, conv, profile)
+        del conv
+        net, hist = warmstart.train(tenet, X, Y, epochs=epochs, batch=32,
+                                    log=lambda m: log(m))
+        del X, Y
+        t_train = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        starts = warmstart.predict_fields(tenet, net, pts, profile=profile)
+        t_inf = time.perf_counter() - t0
+        net.free()
+        info = {'train_points': learned, 'train_converged': int(res_tr.converged.sum()),
+                'profile_fields_used': n_prof, 'net_fields_used': len(keep),
+                'train_data_s': t_data, 'train_s': t_train, 'inference_s': t_inf,
+                'train_loss': [float(hist[0]), float(hist[-1])],
+                'profile_starts': int(sum(p is not None for p in
+                                          profile.predict(pts, [float(np.sum(sweep.cold_start(q)))
+                                                                for q in pts])))}
+    res, fields = sweep.run_sweep(tenet, pts, start=starts, mode=mode, max_steps=max_steps,
+                                  **kw, progress=log)
     out = {'n_points': n_points, 'L': [64, Lmax], 'sites': int(sum(p.L * p.L for p in pts)),
            'mode': mode, 'precision_batch': prec, 'tune_s': t_tune, 'launch': l_full,
            'ms_per_step_all_live': tim[','.join(map(str, l_full))],
-           'launch_small': l_small, 'wall_s': res.wall_s, 'gpu_ms': res.gpu_ms,
+           'launch_small': l_small, 'wall_s': res.wall_s + t_inf, 'gpu_ms': res.gpu_ms,
+           'start': 'learned' if learned else 'cold', 'learned': info,
+           'fallback_reasons': res.reset_counts(),
            'steps_mean': float(res.steps.mean()), 'steps_max': int(res.steps.max()),
            'converged': int(res.converged.sum()), 'max_steps': max_steps,
            'history': res.history[::max(1, len(res.history) // 16)] + res.history[-1:]}
@@ -609,6 +645,9 @@ def main():
                     help='step budget per point of --sweep-batch-only')
     ap.add_argument('--sweep-plain-sample', type=int, default=0, metavar='K',
                     help='with --sweep-batch-only: plain per-point runs of K points')
+    ap.add_argument('--sweep-learned', type=int, default=0, metavar='M',
+                    help='with --sweep-batch-only: start from the learned warm start, '
+                         'fitted on M other points of the same sizes')
     ap.add_argument('--sweep-train', type=int, default=512)
     ap.add_argument('--sweep-eval', type=int, default=128)
     ap.add_argument('--sweep-Lmax', type=int, default=256)
@@ -655,7 +694,8 @@ def main():
         return
     if args.sweep_batch_only:
         r = run_sweep_batch_only(args.sweep_batch_only, args.sweep_Lmax, args.sweep_mode,
-                                 args.sweep_max_steps, args.sweep_plain_sample)
+                                 args.sweep_max_steps, args.sweep_plain_sample,
+                                 args.sweep_learned, args.sweep_epochs)
         print(json.dumps({'sweep_batch_only': r}), flush=True)
         return
     if args.sweep:
