@@ -2,6 +2,7 @@
 the droplet end-state classifier, the Laplace pressure probe and the batch
 autotuner's candidate launches."""
 import numpy as np
+import pytest
 
 
 def test_sample_points_ranges():
@@ -51,3 +52,18 @@ def test_batch_candidates_grids():
     cands = tune.batch_candidates(256, items)
     extra = [c for c in cands if c not in base]
     assert sorted(extra) == sorted(s + (5000,) for s in shapes if s[0] in (1, 3, 4))
+
+
+def test_droplet_centre_offsets():
+    from idmx import configs, sweep
+    for c, want in (((32.0, 32.0), (0.0, 0.0)), ((31.5, 32.5), (-0.5, 0.5)),
+                    ((33.0, 30.0), (1.0, -2.0))):
+        n = configs.droplet((64, 64), 12.0, 2.0, centre=c)
+        got = sweep.droplet_centre(n)
+        assert abs(got[0] - want[0]) < 1e-6 and abs(got[1] - want[1]) < 1e-6, (c, got)
+
+
+def test_run_sweep_rejects_unknown_rest_init():
+    from idmx import sweep
+    with pytest.raises(ValueError):
+        sweep.run_sweep(None, [sweep.SweepPoint(L=16, G=-1.6, R=4.0)], rest_init='hot')
