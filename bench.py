@@ -262,6 +262,7 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     conv_fields = [f if c else None for f, c in zip(f_tr, res_tr.converged)]
     profile = warmstart.ProfileModel()
     out['profile_fields_used'] = profile.fit(train_pts, conv_fields)
+    out['uniform_fields_used'] = profile.n_uniform
     X, Y, keep = warmstart.training_set(train_pts, conv_fields, profile)
     out['train_fields_used'] = len(keep)
     net, hist = warmstart.train(tenet, X, Y, epochs=epochs, batch=32, log=log)
@@ -280,23 +281,27 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     masses = [float(np.sum(sweep.cold_start(p))) for p in eval_pts]
     prof_starts = profile.predict(eval_pts, masses)
     t_prof = time.perf_counter() - t0
-    # learned starts run under the default fallback controller (mass screen,
-    # non-finite reset, growth rule: 5 checks above 10x the best residual)
-    # with populations at rest under the Guo forcing; the last three runs
-    # are ablations: the profile model alone (no net), the reference's
-    # f = w n populations, and the growth rule off
-    runs = {'batched_cold_fp64': (None, 'fp64', 5, 'force', 0.0),
-            'batched_cold': (None, 'fp32+fp64', 5, 'force', 0.0),
-            'batched_warm_fp64': (starts, 'fp64', 5, 'force', t_inf),
-            'batched_warm': (starts, 'fp32+fp64', 5, 'force', t_inf),
-            'batched_warm_fp64_profile_only': (prof_starts, 'fp64', 5, 'force', t_prof),
-            'batched_warm_fp64_plain_populations': (starts, 'fp64', 5, 'plain', t_inf),
-            'batched_warm_fp64_no_growth_rule': (starts, 'fp64', None, 'force', t_inf)}
+    out['starts_uniform'] = int(sum(profile.is_uniform(p, m) for p, m in zip(eval_pts, masses)))
+    # every run uses the default fallback controller (mass screen, non-finite
+    # reset, growth rule: 5 checks above 10x the best residual; a diverging
+    # cold start restarts once from the gentle start) and writes learned
+    # starts at rest under the Guo forcing; the last four runs are
+    # ablations: the cold start without the gentle restart (the plain runs'
+    # rules: same steps, same points), the profile model alone (no net), the
+    # reference's f = w n populations, and the growth rule off
+    runs = {'batched_cold_fp64': (None, 'fp64', 5, 'force', 0.0, True),
+            'batched_cold': (None, 'fp32+fp64', 5, 'force', 0.0, True),
+            'batched_warm_fp64': (starts, 'fp64', 5, 'force', t_inf, True),
+            'batched_warm': (starts, 'fp32+fp64', 5, 'force', t_inf, True),
+            'batched_cold_fp64_no_gentle': (None, 'fp64', 5, 'force', 0.0, False),
+            'batched_warm_fp64_profile_only': (prof_starts, 'fp64', 5, 'force', t_prof, True),
+            'batched_warm_fp64_plain_populations': (starts, 'fp64', 5, 'plain', t_inf, True),
+            'batched_warm_fp64_no_growth_rule': (starts, 'fp64', None, 'force', t_inf, True)}
     fields = {}
-    for key, (st, mode, gc, rest, t_start) in runs.items():
+    for key, (st, mode, gc, rest, t_start, gentle) in runs.items():
         res, f = sweep.run_sweep(tenet, eval_pts, start=st, mode=mode, launch=l64,
                                  launch32=l32, launch_small=s64, launch32_small=s32,
-                                 grow_checks=gc, rest_init=rest)
+                                 grow_checks=gc, rest_init=rest, gentle=gentle)
         fields[key] = f
         wall = res.wall_s + t_start
         out[key] = {'wall_s': wall, 'gpu_ms': res.gpu_ms, 'steps_mean': float(res.steps.mean()),

@@ -29,6 +29,11 @@ its trajectory would repeat):
     monotone) from triggering it.
 A reset domain gets a fresh step budget, so resets cost time but not
 converged points. SweepResult.reason records why each point fell back.
+A cold start that turns non-finite (near the weak end of the G range its
+sharp interface drives populations negative) is restarted once from
+gentle_start, the same mass spread almost uniformly (run_sweep's `gentle`;
+SweepResult.gentle marks those points); it is dropped as failed only if that
+diverges too.
 """
 import ctypes as C
 import time
@@ -64,15 +69,29 @@ class SweepResult:
     gpu_ms: float
     history: list = field(default_factory=list)
     reason: np.ndarray = None   # FB_* per point
+    gentle: np.ndarray = None   # restarted from gentle_start after the cold start diverged
 
     def reset_counts(self):
         r = self.reason if self.reason is not None else np.zeros(len(self.steps), dtype=int)
+        g = self.gentle if self.gentle is not None else np.zeros(len(self.steps), dtype=bool)
         return {'nonfinite': int((r == FB_NONFINITE).sum()), 'growth': int((r == FB_GROWTH).sum()),
-                'mass': int((r == FB_MASS).sum())}
+                'mass': int((r == FB_MASS).sum()), 'gentle': int(g.sum())}
 
 
 def cold_start(p, W=2.0):
     return droplet((p.L, p.L), p.R, W)
+
+
+def gentle_start(p, amp=0.05, W=2.0):
+    """The cold start's mass spread almost uniformly: its mean density plus
+    a centred bump of the cold droplet's shape, peak-to-peak `amp`. Near the
+    weak end of the G range the cold start's sharp 0.1 / 2.0 interface drives
+    the populations negative within ~2000 steps, although the steady state
+    there is often a uniform gas; from this start the dynamics stay small."""
+    n0 = cold_start(p, W)
+    shape = (n0 - n0.min()) / (n0.max() - n0.min())
+    shape = shape - shape.mean()
+    return n0.mean() + amp * shape / (shape.max() - shape.min())
 
 
 class Batch:
@@ -194,7 +213,7 @@ def _apply_launch(batch, active, launch, launch_small):
 
 def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks, hist,
            restart, label, plateau=False, launch=None, launch_small=None, plateau_checks=5,
-           progress=None, reason=None):
+           progress=None, reason=None, restart_gentle=None, gentled=None):
     """Runs the live domains of `batch` until each one's max |dn| < tol, or
     until every live domain has run max_steps steps since its (re)start in
     this call. A domain on a caller-supplied start (warm[i]) that turns
@@ -203,7 +222,10 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
     `grow_checks` consecutive checks, is handed to restart(i) (the cold-start
     fallback) once, with a fresh step budget; reason[i] (if given) records
     FB_NONFINITE or FB_GROWTH. A cold domain that turns non-finite is
-    dropped as failed. With plateau=True (the FP32 phase) a domain also
+    dropped as failed, unless restart_gentle is given: then it is handed to
+    restart_gentle(i) once (gentled[i] records it), with a fresh step
+    budget, and fails only if it turns non-finite again. With plateau=True
+    (the FP32 phase) a domain also
     finishes once its residual has not improved by 20 % for `plateau_checks`
     checks: FP32 rounding sets a floor below which it cannot go, and the
     FP64 phase takes over there.
@@ -246,6 +268,16 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
         grew = grows >= grow_checks if grow_checks is not None else np.zeros(n, dtype=bool)
         bad = active & ~done & (~fin | (warm & grew))
         dead = bad & ~warm
+        if restart_gentle is not None:
+            soft = dead & ~gentled
+            for i in np.nonzero(soft)[0]:
+                restart_gentle(i)
+                gentled[i] = True
+                grows[i] = stale[i] = 0
+                best[i] = np.inf
+                first[i] = np.nan
+                ran[i] = 0
+            dead = dead & ~soft
         failed |= dead
         changed = (done | dead).any()
         active = active & ~done & ~dead
@@ -303,7 +335,8 @@ def screen_starts(points, start, mass_tol=0.05):
 def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
               check_every=200, max_steps=400000, grow_checks=5, estencil='E4',
               launch=None, launch32=None, rho0=1.0, symmetric=True, launch_small=None,
-              launch32_small=None, progress=None, mass_tol=0.05, rest_init='force'):
+              launch32_small=None, progress=None, mass_tol=0.05, rest_init='force',
+              gentle=True):
     """Relax every point to steady state; returns SweepResult and the final
     FP64 densities. start: None (cold) or a list of initial densities (None
     entries start cold). mode: 'fp64' (FP64 only) or 'fp32+fp64'.
@@ -321,7 +354,10 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     start and at the FP32 -> FP64 hand-over: 'force' (default) the fluid at
     rest under the Guo forcing, f = f_eq(n, -F/(2n)) (Batch.force_rest);
     'plain' f = w n, as the cold start and the reference's InitPopulations
-    at u = 0, which start every interface at the velocity F/(2n)."""
+    at u = 0, which start every interface at the velocity F/(2n).
+    gentle: a domain whose cold start turns non-finite is restarted once from
+    gentle_start (same mass) instead of being dropped; SweepResult.gentle
+    marks those points."""
     if rest_init not in ('force', 'plain'):
         raise ValueError("rest_init is 'force' or 'plain'")
     n = len(points)
@@ -335,8 +371,15 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
         b64.set_launch(*launch)
     gpu_ms = 0.0
 
+    gentled = np.zeros(n, dtype=bool)
+
     def restart64(i):
         n0 = cold_start(points[i])
+        mass0[i] = float(np.sum(n0))
+        b64.set_density(i, n0)
+
+    def gentle64(i):
+        n0 = gentle_start(points[i])
         mass0[i] = float(np.sum(n0))
         b64.set_density(i, n0)
 
@@ -356,10 +399,16 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
             mass0[i] = float(np.sum(n0))
             b32.set_density(i, n0)
 
+        def gentle32(i):
+            n0 = gentle_start(points[i])
+            mass0[i] = float(np.sum(n0))
+            b32.set_density(i, n0)
+
         _, failed, fb, ms = _relax(b32, np.ones(n, dtype=bool), warm, steps, tol32, check_every,
                                    max_steps, grow_checks, hist, restart32, 'fp32',
                                    plateau=True, launch=launch32, launch_small=launch32_small,
-                                   progress=progress, reason=reason)
+                                   progress=progress, reason=reason,
+                                   restart_gentle=gentle32 if gentle else None, gentled=gentled)
         gpu_ms += ms
         fb_all |= fb
         # FP32 rounding leaves each domain a small net momentum that FP64
@@ -374,7 +423,7 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
         if rest_init == 'force':
             b64.force_rest(-1)
         for i in np.nonzero(failed)[0]:   # FP32 blew up: FP64 from the cold start
-            restart64(i)
+            (gentle64 if gentled[i] else restart64)(i)
             fb_all[i] = True
     elif mode == 'fp64':
         for i in range(n):
@@ -388,7 +437,8 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     rem, failed, fb, ms = _relax(b64, np.ones(n, dtype=bool), warm, steps, tol, check_every,
                                  max_steps, grow_checks, hist, restart64, 'fp64',
                                  launch=launch, launch_small=launch_small, progress=progress,
-                                 reason=reason)
+                                 reason=reason, restart_gentle=gentle64 if gentle else None,
+                                 gentled=gentled)
     gpu_ms += ms
     fb_all |= fb
     fail_all |= failed
@@ -396,7 +446,7 @@ def run_sweep(tenet, points, start=None, mode='fp64', tol=1e-10, tol32=1e-6,
     b64.close()
     wall = time.perf_counter() - t0
     return SweepResult(steps=steps, converged=~rem & ~fail_all, fallback=fb_all, wall_s=wall,
-                       gpu_ms=gpu_ms, history=hist, reason=reason), fields
+                       gpu_ms=gpu_ms, history=hist, reason=reason, gentle=gentled), fields
 
 
 def _shift_to_mass(batch, i, mass0, symmetric):

@@ -391,6 +391,9 @@ def predict_fields(tenet, net, points, cold_mass=None, profile=None):
         imgs = net.predict_images(xs)
         for j, p in enumerate(chunk):
             lj = lat[i + j]
+            if profile is not None and profile.is_uniform(p, masses[i + j]):
+                out.append(lj)          # already steady: no correction
+                continue
             img = imgs[j] if lj is not None else bases[j] + imgs[j]
             img = np.ascontiguousarray(img, dtype=np.float32)
             dimg = _Buf(tenet, img.nbytes)
@@ -430,10 +433,47 @@ class ProfileModel:
 
     The model sees the steady state at lattice resolution, which the U-Net's
     S x S image cannot (one pixel covers L / S sites, several interface
-    widths at L = 256)."""
+    widths at L = 256).
+
+    Training fields that relaxed to a uniform gas (near the weak end of the G
+    range a droplet of the sweep's mass evaporates) train a logistic
+    classifier of droplet vs uniform on (G, mean density, 1/L); a point it
+    calls uniform with probability above P_UNIFORM starts as the exactly
+    uniform field, which is already steady."""
+
+    P_UNIFORM = 0.95
 
     S_MAX = 6.0
     NB = 121
+
+    @staticmethod
+    def _cls_feat(G, nu, L):
+        g = (G + 1.6) / 0.25
+        return np.array([1.0, g, nu, g * nu, nu * nu, g * g, 64.0 / L])
+
+    @staticmethod
+    def _logistic(X, y, ridge=1e-3, iters=50):
+        """Logistic regression by Newton steps (IRLS) with a small ridge."""
+        w = np.zeros(X.shape[1])
+        for _ in range(iters):
+            p = 1.0 / (1.0 + np.exp(-np.clip(X @ w, -40, 40)))
+            H = X.T @ (X * (p * (1 - p))[:, None]) + ridge * np.eye(X.shape[1])
+            step = np.linalg.solve(H, X.T @ (p - y) + ridge * w)
+            w -= step
+            if np.abs(step).max() < 1e-10:
+                break
+        return w
+
+    def p_uniform(self, G, nu, L):
+        """Probability that a point's steady state is a uniform gas (0 when
+        the training fields held too few of either kind)."""
+        if self.cls_coef is None:
+            return 0.0
+        return float(1.0 / (1.0 + np.exp(-np.clip(self._cls_feat(G, nu, L) @ self.cls_coef,
+                                                   -40, 40))))
+
+    def is_uniform(self, p, mass):
+        return self.p_uniform(p.G, mass / (p.L * p.L), p.L) > self.P_UNIFORM
 
     @staticmethod
     def _feat(G, R):
@@ -478,13 +518,23 @@ class ProfileModel:
         """points: SweepPoint-like (G, L); fields: their equilibrated
         densities (None or non-droplet entries are skipped). Returns the
         number of fields used."""
-        rows = []
+        rows, cls_x, cls_y = [], [], []
         for p, f in zip(points, fields):
             if f is None or not np.isfinite(f).all():
                 continue
+            f = np.asarray(f, dtype=np.float64).ravel()
             d = self.describe(f, p.L)
             if d is not None:
                 rows.append((p.G,) + d)
+                cls_x.append(self._cls_feat(p.G, f.mean(), p.L))
+                cls_y.append(0.0)
+            elif np.ptp(f) <= 1e-6 * abs(f.mean()):
+                cls_x.append(self._cls_feat(p.G, f.mean(), p.L))
+                cls_y.append(1.0)
+        self.cls_coef = None
+        self.n_uniform = int(sum(cls_y))
+        if self.n_uniform >= 4 and len(cls_y) - self.n_uniform >= 4:
+            self.cls_coef = self._logistic(np.array(cls_x), np.array(cls_y))
         if len(rows) < 12:
             raise ValueError('ProfileModel.fit: %d usable fields, need 12' % len(rows))
         X = np.stack([self._feat(r[0], r[3]) for r in rows])
@@ -510,12 +560,16 @@ class ProfileModel:
         return n_g + (n_l - n_g) * (0.5 * (1 - np.tanh(s)) + rest)
 
     def predict(self, points, masses):
-        """Start densities of `points` whose totals are `masses`: the radius
-        by Newton steps on the rendered mass, then a uniform shift to the
-        exact mass. None where the mass leaves no droplet of at least three
-        interface widths (the caller starts those cold)."""
+        """Start densities of `points` whose totals are `masses`: the uniform
+        field where the classifier calls the point uniform; else the profile,
+        its radius by Newton steps on the rendered mass, then a uniform shift
+        to the exact mass. None where the mass leaves no droplet of at least
+        three interface widths (the caller starts those otherwise)."""
         out = []
         for p, m in zip(points, masses):
+            if self.is_uniform(p, m):
+                out.append(np.full(p.L * p.L, m / (p.L * p.L)))
+                continue
             R = float(p.R)
             field = None
             for _ in range(30):

@@ -117,3 +117,35 @@ def test_screen_starts_nan_and_mass():
     assert np.array_equal(init[0], cold) and np.array_equal(init[1], cold)
     assert np.array_equal(init[2], 1.02 * cold)
     assert np.allclose(mass0, cold.sum())
+
+
+def test_diverging_cold_start_gets_one_gentle_restart():
+    """A cold domain that turns non-finite is restarted from the gentle start
+    once (fresh budget); one that diverges again is dropped as failed; with
+    no gentle restart given it is dropped at once."""
+    nan_at = lambda j: (lambda k: np.nan if k >= j else 1e-3)
+    for with_gentle in (True, False):
+        b = ScriptedBatch([nan_at(2), nan_at(1), decay(1e-2, 0.3)], cold=None)
+        gentle_seq = [decay(1e-3, 0.3), nan_at(0), None]
+        calls = []
+
+        def gentle(i):
+            calls.append(i)
+            b.seqs[i] = gentle_seq[i]
+            b.k[i] = 0
+
+        n = 3
+        steps = np.zeros(n, dtype=np.int64)
+        gentled = np.zeros(n, dtype=bool)
+        rem, failed, fb, _ = sweep._relax(
+            b, np.ones(n, dtype=bool), np.zeros(n, dtype=bool), steps, 1e-10, 100, 6000, 5, [],
+            b.restart, 't', restart_gentle=gentle if with_gentle else None, gentled=gentled)
+        if with_gentle:
+            assert calls == [1, 0] or sorted(calls) == [0, 1]
+            assert list(gentled) == [True, True, False]
+            assert list(failed) == [False, True, False]
+            assert not rem.any()
+            assert steps[0] > 300      # its budget restarted: it ran on after the reset
+        else:
+            assert calls == [] and list(failed) == [True, True, False]
+        assert not fb.any() and b.restarts == []

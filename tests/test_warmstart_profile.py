@@ -94,3 +94,33 @@ def test_fit_needs_enough_fields():
     pts = _points(5, 6)
     with pytest.raises(ValueError):
         ws.ProfileModel().fit(pts, [_field(p.G, p.L, p.R) for p in pts])
+
+
+def test_uniform_classifier():
+    """Droplets at strong coupling / high mean density, uniform gas at weak
+    coupling / low mean density: the classifier separates them, a uniform
+    point starts exactly uniform at its mass, and a droplet point still gets
+    the profile."""
+    rng = np.random.default_rng(8)
+    pts, fields = [], []
+    for _ in range(80):
+        L = int(rng.choice((64, 96, 128)))
+        if rng.random() < 0.5:
+            p = SweepPoint(L=L, G=float(rng.uniform(-1.8, -1.6)), R=float(rng.uniform(0.2 * L, 0.33 * L)))
+            fields.append(_field(p.G, p.L, p.R))
+        else:
+            p = SweepPoint(L=L, G=float(rng.uniform(-1.5, -1.4)), R=float(rng.uniform(0.1, 0.15) * L))
+            fields.append(np.full(L * L, float(rng.uniform(0.15, 0.3))))
+        pts.append(p)
+    pm = ws.ProfileModel()
+    pm.fit(pts, fields)
+    assert pm.cls_coef is not None and pm.n_uniform >= 20
+    gas = SweepPoint(L=96, G=-1.45, R=12.0)
+    m_gas = 0.2 * 96 * 96
+    st = pm.predict([gas], [m_gas])[0]
+    assert pm.is_uniform(gas, m_gas)
+    assert np.ptp(st) == 0.0 and abs(st.sum() - m_gas) < 1e-9 * m_gas
+    drop = SweepPoint(L=96, G=-1.7, R=25.0)
+    m_drop = _field(drop.G, drop.L, drop.R).sum()
+    assert not pm.is_uniform(drop, m_drop)
+    assert np.ptp(pm.predict([drop], [m_drop])[0]) > 1.0
