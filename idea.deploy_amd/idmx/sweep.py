@@ -235,6 +235,8 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
     every 50 checks.
     Returns (still active, failed, fell back, GPU ms)."""
     n = batch.n
+    if restart_gentle is not None and gentled is None:
+        gentled = np.zeros(n, dtype=bool)
     best = np.full(n, np.inf)
     first = np.full(n, np.nan)   # residual at the first check after the start
     stale = np.zeros(n, dtype=int)
