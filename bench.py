@@ -220,7 +220,8 @@ def _cpu_sample(xi, es, dims, n0, budget_s):
             'lattice': 'x'.join(map(str, dims)), 'stencil': es}
 
 
-def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
+def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print, train_tol=1e-8,
+                    train_max_steps=400000, ablations=True):
     """Time to steady state (|dn|_max < 1e-10 between checks 200 steps apart)
     of a (G, R, L) sweep of centred droplets: plain per-point FP64 runs from
     the cold start vs batched runs (FP64, and FP32 then FP64) from the cold
@@ -228,7 +229,10 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     G uniform in [-5.5, -4.2] (textbook normalisation, SC_G = G/3), R in
     [0.1L, 0.35L]. The warm-start net is trained first on equilibrated fields
     of other points produced by the batched solver; the batch launch shapes
-    come from the autotuner (its time is reported, not counted)."""
+    come from the autotuner (its time is reported, not counted). train_tol /
+    train_max_steps: the training sweep's tolerance and step budget (above
+    4096 training points it runs on the evaluation batch's tuned launch);
+    ablations=False runs the four main batched runs only."""
     from idmx import configs, sweep, tune, warmstart
     from idmx.IdpyCode import Tenet
     tenet = Tenet(0)
@@ -249,10 +253,15 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
     log('batch launch fp64 %s %.4f ms/step, fp32s %s %.4f ms/step' % (
         l64, out['launch_fp64']['ms_per_step'], l32, out['launch_fp32s']['ms_per_step']))
     t0 = time.perf_counter()
-    lt, _ = tune.tune_batch(tenet, train_pts, 'fp64', use_cache=False)
-    lts = tune.tune_batch_small(tenet, train_pts, 'fp64')
+    if n_train > 4096:
+        lt, lts = l64, s64
+    else:
+        lt, _ = tune.tune_batch(tenet, train_pts, 'fp64', use_cache=False)
+        lts = tune.tune_batch_small(tenet, train_pts, 'fp64')
+    out['train_tol'], out['train_max_steps'] = train_tol, train_max_steps
     res_tr, f_tr = sweep.run_sweep(
-        tenet, train_pts, mode='fp64', tol=1e-8, launch=lt, launch_small=lts,
+        tenet, train_pts, mode='fp64', tol=train_tol, max_steps=train_max_steps, launch=lt,
+        launch_small=lts,
         progress=lambda lab, st, live, dn: log('training sweep: %d steps, %d live, max |dn| %.1e'
                                                % (st, live, dn)))
     out['train_data_s'] = time.perf_counter() - t0
@@ -297,6 +306,9 @@ def run_sweep_study(n_train, n_eval, Lmax, epochs, log=print):
             'batched_warm_fp64_profile_only': (prof_starts, 'fp64', 5, 'force', t_prof, True),
             'batched_warm_fp64_plain_populations': (starts, 'fp64', 5, 'plain', t_inf, True),
             'batched_warm_fp64_no_growth_rule': (starts, 'fp64', None, 'force', t_inf, True)}
+    if not ablations:
+        runs = {k: v for k, v in runs.items()
+                if k in ('batched_cold_fp64', 'batched_cold', 'batched_warm_fp64', 'batched_warm')}
     fields = {}
     for key, (st, mode, gc, rest, t_start, gentle) in runs.items():
         res, f = sweep.run_sweep(tenet, eval_pts, start=st, mode=mode, launch=l64,
@@ -654,6 +666,12 @@ def main():
                     help='with --sweep-batch-only: start from the learned warm start, '
                          'fitted on M other points of the same sizes')
     ap.add_argument('--sweep-train', type=int, default=512)
+    ap.add_argument('--sweep-train-tol', type=float, default=1e-8,
+                    help='tolerance of the training sweep of --sweep')
+    ap.add_argument('--sweep-train-max-steps', type=int, default=400000,
+                    help='step budget per training point of --sweep')
+    ap.add_argument('--no-sweep-ablations', action='store_true',
+                    help='--sweep: the four main batched runs only')
     ap.add_argument('--sweep-eval', type=int, default=128)
     ap.add_argument('--sweep-Lmax', type=int, default=256)
     ap.add_argument('--sweep-epochs', type=int, default=120)
@@ -705,7 +723,10 @@ def main():
         return
     if args.sweep:
         r = run_sweep_study(args.sweep_train, args.sweep_eval, args.sweep_Lmax,
-                            args.sweep_epochs, log=lambda m: print(m, file=sys.stderr, flush=True))
+                            args.sweep_epochs, log=lambda m: print(m, file=sys.stderr, flush=True),
+                            train_tol=args.sweep_train_tol,
+                            train_max_steps=args.sweep_train_max_steps,
+                            ablations=not args.no_sweep_ablations)
         line = {'metric': 'sweep time-to-steady-state learned vs plain',
                 'value': round(r['speedup_warm_vs_plain'], 3), 'unit': 'x (plain / learned)',
                 'higher_is_better': True, 'n_gpus': 1, 'data': 'synthetic',

@@ -232,7 +232,7 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
     batch needs set_active, run(nsteps) -> GPU ms and check() -> max |dn|
     per domain (NaN or inf for a non-finite state). launch / launch_small:
     see _apply_launch. progress(label, steps, live, max |dn|) is called
-    every 50 checks.
+    every 50 checks, and after any check 30 s or more after the last call.
     Returns (still active, failed, fell back, GPU ms)."""
     n = batch.n
     if restart_gentle is not None and gentled is None:
@@ -245,6 +245,7 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
     failed = np.zeros(n, dtype=bool)
     ran = np.zeros(n, dtype=np.int64)
     gpu_ms = 0.0
+    t_prog = time.perf_counter()
     active = active.copy()
     batch.set_active(active)
     _apply_launch(batch, active, launch, launch_small)
@@ -265,7 +266,9 @@ def _relax(batch, active, warm, steps, tol, check_every, max_steps, grow_checks,
             done |= active & fin & (stale >= plateau_checks)
         hist.append((label, int(steps[active].max()), int(active.sum()),
                      float(np.max(np.where(active & fin, dn, 0.0)))))
-        if progress is not None and len(hist) % 50 == 0:
+        if progress is not None and (len(hist) % 50 == 0
+                                     or time.perf_counter() - t_prog > 30.0):
+            t_prog = time.perf_counter()
             progress(*hist[-1])
         grew = grows >= grow_checks if grow_checks is not None else np.zeros(n, dtype=bool)
         bad = active & ~done & (~fin | (warm & grew))
