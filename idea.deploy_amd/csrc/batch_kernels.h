@@ -257,102 +257,131 @@ struct Tile2Shape {
   static constexpr int K = (NL + NT - 1) / NT;                  // L0 sites per thread
 };
 
+// Phase A's loads: f(0) of the item's L0 sites into registers (each thread
+// keeps the values of its L0 sites).
+template <class T, int BX, int BY>
+__device__ __forceinline__ void tile2_load(const T* __restrict__ pop, const BatchItem& w,
+                                           const BatchDom& d,
+                                           T (&f0)[Tile2Shape<BX, BY>::K][D2Q9::Q]) {
+  using SH = Tile2Shape<BX, BY>;
+  const int64_t V = (int64_t)d.L0 * d.L1;
+#pragma unroll
+  for (int k = 0; k < SH::K; k++) {
+    const int sidx = threadIdx.x + k * SH::NT;
+    if (sidx < SH::NL) {
+      const int lx = sidx % SH::LW, ly = sidx / SH::LW;
+      const int xs = pmod(w.x0 - 3 + lx, d.L0), ys = pmod(w.m0 - 3 + ly, d.L1);
+      const int64_t g = (int64_t)ys * d.L0 + xs;
+#pragma unroll
+      for (int q = 0; q < D2Q9::Q; q++) f0[k][q] = pop[(size_t)q * V + g];
+    }
+  }
+}
+
+// Phases A (from the loaded values) to D of one item. The barriers wait for
+// LDS only: no global memory written in a launch is read in it, and the
+// push stores stay in flight through the next barrier.
 template <class T, bool SHIFT, int BX, int BY>
-__global__ void __launch_bounds__(BX * BY)
-k_batched_tile2(const T* __restrict__ pop_all, T* __restrict__ swap_all,
-                const BatchDom* __restrict__ doms, const BatchItem* __restrict__ items,
-                const int* __restrict__ n_items) {
+__device__ __forceinline__ void tile2_body(
+    const T (&f0)[Tile2Shape<BX, BY>::K][D2Q9::Q], const BatchItem& w, const BatchDom& d,
+    T* __restrict__ swap_all, T (*a0)[Tile2Shape<BX, BY>::NC],
+    T (*psi0)[Tile2Shape<BX, BY>::LW], T (*psi1)[Tile2Shape<BX, BY>::PW]) {
   using S = D2Q9;
   using ES = D2E4;
   using SH = Tile2Shape<BX, BY>;
   constexpr int Q = S::Q, NT = SH::NT, K = SH::K;
   static_assert(ES::R == 1, "two steps per pass: reach-1 force stencil");
-  __shared__ T a0[Q][SH::NC];      // f*(0) on C0
-  __shared__ T psi0[SH::LH][SH::LW];
-  __shared__ T psi1[SH::PH][SH::PW];
   const int tid = threadIdx.x;
   const int tx = tid % BX, ty = tid / BX;
-  const int ni = *n_items;
-  for (int it = xcd_block(); it < ni; it += gridDim.x) {
-    const BatchItem w = items[it];
-    const BatchDom d = doms[w.dom];
-    const Geom geo{d.L0, d.L1, 1, (int64_t)d.L0 * d.L1};
-    const Phys<T> P = dom_phys<T>(d);
-    const T* pop = pop_all + d.pop_off;
-    // phase A: L0 = [x0 - 3, x0 + BX + 3) x [y0 - 3, y0 + BY + 3)
-    T f0[K][Q];
-    T s0[K];
+  const Geom geo{d.L0, d.L1, 1, (int64_t)d.L0 * d.L1};
+  const Phys<T> P = dom_phys<T>(d);
+  // phase A: n(0), psi(0) of L0 = [x0 - 3, x0 + BX + 3) x [y0 - 3, y0 + BY + 3)
+  T s0[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-      const int sidx = tid + k * NT;
-      if (sidx < SH::NL) {
-        const int lx = sidx % SH::LW, ly = sidx / SH::LW;
-        const int xs = pmod(w.x0 - 3 + lx, d.L0), ys = pmod(w.m0 - 3 + ly, d.L1);
-        const int64_t g = (int64_t)ys * d.L0 + xs;
-#pragma unroll
-        for (int q = 0; q < Q; q++) f0[k][q] = pop[(size_t)q * geo.V + g];
-        T s = 0;
-#pragma unroll
-        for (int q = 0; q < Q; q++) s += f0[k][q];
-        s0[k] = s;
-        psi0[ly][lx] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
-      }
-    }
-    __syncthreads();
-    // phase B: the C0 sites among this thread's L0 sites
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const int sidx = tid + k * NT;
+  for (int k = 0; k < K; k++) {
+    const int sidx = tid + k * NT;
+    if (sidx < SH::NL) {
       const int lx = sidx % SH::LW, ly = sidx / SH::LW;
-      if (sidx < SH::NL && lx >= 1 && lx < SH::LW - 1 && ly >= 1 && ly < SH::LH - 1) {
-        T gx = 0, gy = 0;
-#pragma unroll
-        for (int e = 0; e < ES::QE; e++) {
-          const T v = psi0[ly + ES::ey[e]][lx + ES::ex[e]];
-          const T wgt = T(ES::w[e]);
-          if (ES::ex[e] != 0) gx += T(ES::ex[e]) * wgt * v;
-          if (ES::ey[e] != 0) gy += T(ES::ey[e]) * wgt * v;
-        }
-        T fo[Q], u[3];
-        collide<S, T, SHIFT, 0>(f0[k], s0[k], gx, gy, T(0), psi0[ly][lx], P, fo, u);
-        const int c = (ly - 1) * SH::CW + (lx - 1);
-#pragma unroll
-        for (int q = 0; q < Q; q++) a0[q][c] = fo[q];
-      }
-    }
-    __syncthreads();
-    // phase C: n(1), psi(1) on the tile plus a 1-site ring (P region)
-    for (int sidx = tid; sidx < SH::NP; sidx += NT) {
-      const int px = sidx % SH::PW, py = sidx / SH::PW;
       T s = 0;
 #pragma unroll
-      for (int q = 0; q < Q; q++)
-        s += a0[q][(py + 1 - S::cy[q]) * SH::CW + (px + 1 - S::cx[q])];
-      psi1[py][px] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+      for (int q = 0; q < Q; q++) s += f0[k][q];
+      s0[k] = s;
+      psi0[ly][lx] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
     }
-    __syncthreads();
-    // phase D: the tile's own sites
-    const int x = w.x0 + tx, y = w.m0 + ty;
-    if (x < d.L0 && y < d.L1) {
-      T f[Q];
+  }
+  lds_barrier();
+  // phase B: the C0 sites among this thread's L0 sites
 #pragma unroll
-      for (int q = 0; q < Q; q++) f[q] = a0[q][(ty + 2 - S::cy[q]) * SH::CW + (tx + 2 - S::cx[q])];
-      T s = 0;
-#pragma unroll
-      for (int q = 0; q < Q; q++) s += f[q];
+  for (int k = 0; k < K; k++) {
+    const int sidx = tid + k * NT;
+    const int lx = sidx % SH::LW, ly = sidx / SH::LW;
+    if (sidx < SH::NL && lx >= 1 && lx < SH::LW - 1 && ly >= 1 && ly < SH::LH - 1) {
       T gx = 0, gy = 0;
 #pragma unroll
       for (int e = 0; e < ES::QE; e++) {
-        const T v = psi1[ty + 1 + ES::ey[e]][tx + 1 + ES::ex[e]];
+        const T v = psi0[ly + ES::ey[e]][lx + ES::ex[e]];
         const T wgt = T(ES::w[e]);
         if (ES::ex[e] != 0) gx += T(ES::ex[e]) * wgt * v;
         if (ES::ey[e] != 0) gy += T(ES::ey[e]) * wgt * v;
       }
       T fo[Q], u[3];
-      collide<S, T, SHIFT, 0>(f, s, gx, gy, T(0), psi1[ty + 1][tx + 1], P, fo, u);
-      push_store<S, T>(swap_all + d.pop_off, geo, x, y, 0, fo);
+      collide<S, T, SHIFT, 0>(f0[k], s0[k], gx, gy, T(0), psi0[ly][lx], P, fo, u);
+      const int c = (ly - 1) * SH::CW + (lx - 1);
+#pragma unroll
+      for (int q = 0; q < Q; q++) a0[q][c] = fo[q];
     }
-    __syncthreads();
+  }
+  lds_barrier();
+  // phase C: n(1), psi(1) on the tile plus a 1-site ring (P region)
+  for (int sidx = tid; sidx < SH::NP; sidx += NT) {
+    const int px = sidx % SH::PW, py = sidx / SH::PW;
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      s += a0[q][(py + 1 - S::cy[q]) * SH::CW + (px + 1 - S::cx[q])];
+    psi1[py][px] = psi_of(P.psi_kind, SHIFT ? P.rho0 + s : s);
+  }
+  lds_barrier();
+  // phase D: the tile's own sites
+  const int x = w.x0 + tx, y = w.m0 + ty;
+  if (x < d.L0 && y < d.L1) {
+    T f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = a0[q][(ty + 2 - S::cy[q]) * SH::CW + (tx + 2 - S::cx[q])];
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) s += f[q];
+    T gx = 0, gy = 0;
+#pragma unroll
+    for (int e = 0; e < ES::QE; e++) {
+      const T v = psi1[ty + 1 + ES::ey[e]][tx + 1 + ES::ex[e]];
+      const T wgt = T(ES::w[e]);
+      if (ES::ex[e] != 0) gx += T(ES::ex[e]) * wgt * v;
+      if (ES::ey[e] != 0) gy += T(ES::ey[e]) * wgt * v;
+    }
+    T fo[Q], u[3];
+    collide<S, T, SHIFT, 0>(f, s, gx, gy, T(0), psi1[ty + 1][tx + 1], P, fo, u);
+    push_store<S, T>(swap_all + d.pop_off, geo, x, y, 0, fo);
+  }
+  lds_barrier();
+}
+
+template <class T, bool SHIFT, int BX, int BY>
+__global__ void __launch_bounds__(BX * BY)
+k_batched_tile2(const T* __restrict__ pop_all, T* __restrict__ swap_all,
+                const BatchDom* __restrict__ doms, const BatchItem* __restrict__ items,
+                const int* __restrict__ n_items) {
+  using SH = Tile2Shape<BX, BY>;
+  __shared__ T a0[D2Q9::Q][SH::NC];      // f*(0) on C0
+  __shared__ T psi0[SH::LH][SH::LW];
+  __shared__ T psi1[SH::PH][SH::PW];
+  const int ni = *n_items;
+  for (int it = xcd_block(); it < ni; it += gridDim.x) {
+    const BatchItem w = items[it];
+    const BatchDom d = doms[w.dom];
+    T f0[SH::K][D2Q9::Q];
+    tile2_load<T, BX, BY>(pop_all + d.pop_off, w, d, f0);
+    tile2_body<T, SHIFT, BX, BY>(f0, w, d, swap_all, a0, psi0, psi1);
   }
 }
 

@@ -2,7 +2,8 @@
 populations kept in LDS for the tile plus a 2-site ring) against the
 one-step tiles (kind 1) and the oracle: the same arithmetic in the same
 order, so the densities agree bit for bit; odd step counts, domains smaller
-than a tile, ragged last tiles, inactive domains, FP64 and FP32-shifted."""
+than a tile, ragged last tiles, persistent grids of fewer blocks than items
+(down to one), inactive domains, FP64 and FP32-shifted."""
 import numpy as np
 import pytest
 
@@ -31,7 +32,8 @@ def test_two_steps_per_launch_equals_one_step_tiles(tenet, precision):
         b.run(nsteps)
         ref[nsteps] = [b.get_density(i) for i in range(len(pts))]
     items = {}
-    for shape in [(4, 32, 8, 0), (4, 16, 16, 0), (4, 32, 16, 0), (4, 32, 8, 7), (4, 16, 16, 1000)]:
+    for shape in [(4, 32, 8, 0), (4, 16, 16, 0), (4, 32, 16, 0), (4, 32, 8, 7), (4, 16, 16, 1000),
+                  (4, 32, 16, 13), (4, 16, 16, 1)]:
         for nsteps in (150, 41, 1, 2):
             b.set_launch(*shape)
             for i, n0 in enumerate(starts):
@@ -41,17 +43,18 @@ def test_two_steps_per_launch_equals_one_step_tiles(tenet, precision):
                 got = b.get_density(i)
                 assert np.array_equal(got, ref[nsteps][i]), \
                     (shape, nsteps, i, rel_err(got.ravel(), ref[nsteps][i].ravel()))
-        print(precision, 'kind 4 shape', shape, 'identical to kind 1', flush=True)
+        print(precision, 'shape', shape, 'identical to kind 1', flush=True)
     b.close()
 
 
-def test_two_steps_keep_inactive_domains_and_match_oracle(tenet):
+@pytest.mark.parametrize('launch', [(4, 32, 16, 0), (4, 32, 8, 1024)])
+def test_two_steps_keep_inactive_domains_and_match_oracle(tenet, launch):
     from oracle import cpuref
     from idmx.sweep import Batch, cold_start
     pts = _points()
     starts = [cold_start(p) + 1e-4 * np.cos(np.arange(p.L * p.L)) for p in pts]
     b = Batch(tenet, pts, 'fp64')
-    b.set_launch(4, 32, 16, 0)
+    b.set_launch(*launch)
     for i, n0 in enumerate(starts):
         b.set_density(i, n0)
     b.set_active([1, 1, 0, 1])
@@ -67,7 +70,7 @@ def test_two_steps_keep_inactive_domains_and_match_oracle(tenet):
         cpuref.init_pop(st, n0, np.zeros(2 * p.L * p.L))
         cpuref.meta_run(st, 'D2E4', p.G, p.tau, nsteps=40)
         err = rel_err(b.get_density(i).ravel(), st.pop.reshape(9, -1).sum(axis=0))
-        print('kind 4 vs oracle, domain', i, 'L', p.L, err, flush=True)
+        print('kind', launch[0], 'vs oracle, domain', i, 'L', p.L, err, flush=True)
         assert err <= 1e-12
     b.close()
 
