@@ -446,6 +446,10 @@ class ProfileModel:
     S_MAX = 6.0
     NB = 121
 
+    def __init__(self):
+        self.coef = self.bin_coef = self.cls_coef = None
+        self.n_used = self.n_uniform = 0
+
     @staticmethod
     def _cls_feat(G, nu, L):
         g = (G + 1.6) / 0.25
@@ -552,6 +556,8 @@ class ProfileModel:
 
     def render(self, G, L, R):
         """The learned profile at centre radius R on an L x L lattice (flat)."""
+        if self.coef is None:
+            raise RuntimeError('ProfileModel: fit() first')
         f = self._feat(G, R)
         n_l, n_g, W = f @ self.coef
         s = (_centre_dist(L) - R) / W
@@ -565,6 +571,8 @@ class ProfileModel:
         its radius by Newton steps on the rendered mass, then a uniform shift
         to the exact mass. None where the mass leaves no droplet of at least
         three interface widths (the caller starts those otherwise)."""
+        if self.coef is None:
+            raise RuntimeError('ProfileModel: fit() first')
         out = []
         for p, m in zip(points, masses):
             if self.is_uniform(p, m):

@@ -90,6 +90,15 @@ def test_predict_declines_impossible_mass():
     assert pm.predict([p], [2.5 * 64 * 64])[0] is None       # above the liquid density
 
 
+def test_unfitted_model_refuses():
+    pm = ws.ProfileModel()
+    with pytest.raises(RuntimeError):
+        pm.predict([SweepPoint(L=64, G=-1.6, R=15.0)], [1000.0])
+    with pytest.raises(RuntimeError):
+        pm.render(-1.6, 64, 15.0)
+    assert pm.p_uniform(-1.6, 0.3, 64) == 0.0
+
+
 def test_fit_needs_enough_fields():
     pts = _points(5, 6)
     with pytest.raises(ValueError):
