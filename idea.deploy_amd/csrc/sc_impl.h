@@ -163,6 +163,22 @@ struct Impl {
     }
   }
 
+  // The Shan-Chen forcing (COLL 3) and the isotropy filter (COLL 4) on one
+  // fixed fused launch (push form, populations held in registers; rows of
+  // 256 sites on 2D lattices at least that wide, else 64; 32 x 8 tiles in
+  // 3D), not on the autotuner's variants.
+  template <bool WF>
+  static hipError_t fused_fixed(idmx_sc_s* s, const T* pop, T* swap, double* n, double* psi,
+                                double* u) {
+    if constexpr (S::D == 2) {
+      if (!WF && s->geo.L0 >= 256)
+        return launch_fused<256, 1, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
+      return launch_fused<64, 1, WF, true>(s, pop, swap, n, psi, u);
+    } else {
+      return launch_fused<32, 8, WF, true>(s, pop, swap, n, psi, u);
+    }
+  }
+
   // Tile shapes the autotuner chooses from; the step that writes n/psi/u at
   // a checkpoint uses one fixed shape.
   template <bool WF>
@@ -231,7 +247,7 @@ struct Impl {
                         s->desc.psi_kind != IDMX_PSI_PROGRAM &&
                         (s->geo.V >= (int64_t(1) << 19) || (flags & IDMX_RUN_FUSED_WALLS) != 0);
     const bool twop = !fwalls && ((flags & IDMX_RUN_TWO_PASS) != 0 || s->walls || body ||
-                                  COLL >= 3 || s->desc.psi_kind == IDMX_PSI_PROGRAM);
+                                  s->desc.psi_kind == IDMX_PSI_PROGRAM);
     if ((twop || fwalls) && (!n || !psi)) {
       set_error(s->walls || body || s->desc.psi_kind == IDMX_PSI_PROGRAM
                     ? "idmx_sc_run: walls, a body force or a psi program need the n and psi "
@@ -271,7 +287,7 @@ struct Impl {
     // pull-form variants: convert the streamed populations to the pull form,
     // step in it, convert back (nsteps + 2 buffer swaps: same parity as
     // nsteps, so the caller's pointer bookkeeping is unchanged)
-    const bool pull = !twop && !fwalls && pull_form(s->hist) && nsteps > 0;
+    const bool pull = COLL < 3 && !twop && !fwalls && pull_form(s->hist) && nsteps > 0;
     if constexpr (COLL < 3) if (pull) {
       hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
                          (const T*)*pop, (T*)*swap, s->geo, 1);
@@ -286,9 +302,16 @@ struct Impl {
       if (pev) IDMX_HIP(hipEventRecord(pev[2 * k], st));
       hipEvent_t mid = pev ? pev[2 * k + 1] : nullptr;
       if constexpr (COLL >= 3) {
-        // the Shan-Chen forcing and the isotropy filter run on the two-pass
-        // kernels only
-        e = two_pass(s, a, b, n, psi, u, wf && last, mid);
+        // the Shan-Chen forcing and the isotropy filter: the fixed fused
+        // launch, or the two-pass step (walls, a body force, a psi program,
+        // IDMX_RUN_TWO_PASS)
+        if (twop) {
+          e = two_pass(s, a, b, n, psi, u, wf && last, mid);
+        } else {
+          if (mid) IDMX_HIP(hipEventRecord(mid, st));
+          if (wf && last) e = fused_fixed<true>(s, a, b, n, psi, u);
+          else e = fused_fixed<false>(s, a, b, nullptr, nullptr, nullptr);
+        }
       } else {
         if (twop || (fwalls && wf && last)) {
           e = two_pass(s, a, b, n, psi, u, wf && last, mid);
@@ -458,7 +481,7 @@ struct Impl {
 // contraction (its accuracy is set by FP32 storage).
 // coll: 0 BGK, 1 MRT with dense tables, 2 MRT with a diagonal S, 3 BGK with
 // the Shan-Chen forcing, 4 BGK followed by the isotropy filter (D2Q9 and
-// FP64 only; 3 and 4 on the two-pass step only).
+// FP64 only; 3 and 4 on one fixed fused launch or the two-pass step).
 template <class S, class ES>
 const SimOps* ops_for_f64(int coll) {
   if (coll == 4) {

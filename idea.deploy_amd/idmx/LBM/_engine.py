@@ -157,7 +157,7 @@ class SCEngine:
             # the fused step with walls: one kernel per step, the density /
             # psi kernel only in the run's last (two-pass) step
             return ('K_ComputeDensityPsiMeta', 'K_SCStep')
-        if (two_pass or self.forcing == 'sc' or self.iso_filter or body or walls):
+        if two_pass or body or walls or self.psi_kind == 'program':
             return ('K_ComputeDensityPsiMeta',
                     'K_ForceCollideStreamSCMP_MRTMeta' if self.is_mrt
                     else 'K_ForceCollideStreamSCMPMeta')

@@ -60,12 +60,13 @@ enum {
  * the Shan-Chen forcing (equilibrium velocity u + F/(n omega), no Guo term,
  * u written as the mean of the pre- and post-collision velocities;
  * K_Collision_ShanChenMultiPhase[Walls], idpy/LBM/MultiPhaseKernels.py:
- * 132-304, used by MainLoopSimpleSC / MainLoopSimpleWallsSC), which runs
- * on the two-pass step. IDMX_BGK_ISO (D2Q9, FP64 only) is BGK with the Guo
- * forcing followed, on every site, by the isotropy filter of the
- * post-collision populations (K_IsotropyFilter, idpy/LBM/LBMKernelsMeta.py:
- * 895-1080, used by MainLoopIsoFilter, idpy/LBM/MultiPhase.py:1099-1223),
- * also on the two-pass step. */
+ * 132-304, used by MainLoopSimpleSC / MainLoopSimpleWallsSC).
+ * IDMX_BGK_ISO (D2Q9, FP64 only) is BGK with the Guo forcing followed, on
+ * every site, by the isotropy filter of the post-collision populations
+ * (K_IsotropyFilter, idpy/LBM/LBMKernelsMeta.py:895-1080, used by
+ * MainLoopIsoFilter, idpy/LBM/MultiPhase.py:1099-1223). Both run on one
+ * fixed fused launch (not autotuned), and on the two-pass step with walls,
+ * a body force, a psi program or IDMX_RUN_TWO_PASS. */
 enum { IDMX_BGK = 0, IDMX_MRT = 1, IDMX_BGK_SCF = 2, IDMX_BGK_ISO = 3 };
 enum { IDMX_PREC_F64 = 0, IDMX_PREC_F32S = 1 };
 
