@@ -163,19 +163,27 @@ struct Impl {
     }
   }
 
-  // The Shan-Chen forcing (COLL 3) and the isotropy filter (COLL 4) on one
-  // fixed fused launch (push form, populations held in registers; rows of
-  // 256 sites on 2D lattices at least that wide, else 64; 32 x 8 tiles in
-  // 3D), not on the autotuner's variants.
+  // The Shan-Chen forcing (COLL 3) and the isotropy filter (COLL 4) on a
+  // fixed fused launch (push form, populations held in registers), not on
+  // the autotuner's variants: rows of 256 sites on lattices at least that
+  // wide (256 x 2 tiles in 3D), else 64 x 1 (64 x 4); the step that writes
+  // n / psi / u uses 64 x 1 (32 x 8).
   template <bool WF>
   static hipError_t fused_fixed(idmx_sc_s* s, const T* pop, T* swap, double* n, double* psi,
                                 double* u) {
-    if constexpr (S::D == 2) {
-      if (!WF && s->geo.L0 >= 256)
-        return launch_fused<256, 1, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
-      return launch_fused<64, 1, WF, true>(s, pop, swap, n, psi, u);
+    if constexpr (WF) {
+      if constexpr (S::D == 2) return launch_fused<64, 1, true, true>(s, pop, swap, n, psi, u);
+      else return launch_fused<32, 8, true, true>(s, pop, swap, n, psi, u);
     } else {
-      return launch_fused<32, 8, WF, true>(s, pop, swap, n, psi, u);
+      if constexpr (S::D == 2) {
+        if (s->geo.L0 >= 256)
+          return launch_fused<256, 1, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
+        return launch_fused<64, 1, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
+      } else {
+        if (s->geo.L0 >= 256)
+          return launch_fused<256, 2, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
+        return launch_fused<64, 4, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
+      }
     }
   }
 

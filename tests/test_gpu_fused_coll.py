@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 CASES = [('D2Q9', 'D2E4', (96, 70), 'sc', 'fp64'),
          ('D2Q9', 'D2E8', (300, 33), 'sc', 'fp64'),
          ('D3Q19', 'D3E4', (40, 20, 18), 'sc', 'fp64'),
+         ('D3Q19', 'D3E4', (260, 6, 5), 'sc', 'fp64'),
          ('D2Q9', 'D2E4', (64, 48), 'sc', 'fp32s'),
          ('D2Q9', 'D2E4', (64, 37), 'iso', 'fp64'),
          ('D2Q9', 'D2E8', (260, 40), 'iso', 'fp64')]
