@@ -37,7 +37,8 @@ VARIANTS = {0: 'push, re-read', 1: 'push, registers', 2: 'push, re-read + prefet
             3: 'push, re-read, >=3 waves/SIMD', 4: 'push, re-read + prefetch, >=3 waves/SIMD',
             5: 'pull, registers', 6: 'pull, re-read', 7: 'push, re-read, 2 planes/barrier',
             8: 'pull, re-read, 2 planes/barrier',
-            9: 'push, registers, lead plane copied to LDS by DMA'}
+            9: 'push, registers, lead plane copied to LDS by DMA',
+            10: 'variant 9 on the x-shifted storage (line-aligned push stores)'}
 
 WORKLOADS = {
     'd3q19_mrt_256': dict(xi='D3Q19', es='D3E4', dims=(256, 256, 256), coll='mrt',

@@ -45,6 +45,7 @@ struct idmx_sc_s {
   const SimOps* ops = nullptr;
   // fused launch shape (chosen at create time, autotuner may override)
   int tx = 0, ty = 0, H = 0, hist = 1;
+  int xsh = 0;  // variant 10: the running step's x-shifted storage flags
   int coll = 0;  // 0 BGK, 1 dense MRT, 2 diagonal MRT, 3 BGK with Shan-Chen forcing,
                  // 4 BGK followed by the isotropy filter
   const uint8_t* walls = nullptr;  // device flags, 1 fluid / 0 solid (borrowed)

@@ -472,9 +472,9 @@ int idmx_sc_set_launch(idmx_sc_t s, int tx, int ty, int H, int hist) {
                   (tx == 256 && ty == 2),
               "idmx_sc_set_launch: 3D tiles are 32x8, 64x4, 128x2 or 256x2");
   CHECK_ARG(H >= 1, "idmx_sc_set_launch: H must be >= 1");
-  CHECK_ARG(hist >= 0 && hist <= 9, "idmx_sc_set_launch: variant is 0..9");
+  CHECK_ARG(hist >= 0 && hist <= 10, "idmx_sc_set_launch: variant is 0..10");
   CHECK_ARG(s->ops->launch_ok(tx, ty, hist),
-            "idmx_sc_set_launch: the LDS staging of variant 9 does not fit this tile");
+            "idmx_sc_set_launch: the LDS staging of variants 9 and 10 does not fit this tile");
   s->tx = tx;
   s->ty = ty;
   s->H = H;

@@ -70,7 +70,7 @@ struct Impl {
 
   // Lead plane staged in LDS by DMA (variant 9); tiles whose staging buffer
   // does not fit the LDS of a CU are refused.
-  template <int TX, int TY>
+  template <int TX, int TY, bool XSH = false>
   static hipError_t launch_dma(idmx_sc_s* s, const T* pop, T* swap) {
     if constexpr (DmaShape<S, ES, T, TX, TY, COLL>::fits) {
       const Geom& g = s->geo;
@@ -81,9 +81,9 @@ struct Impl {
       const int nseg = (Lm + H - 1) / H;
       const int ntiles = ntx * nty * nseg;
       const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
-      hipLaunchKernelGGL((k_fused_dma<S, ES, T, SHIFT, COLL, TX, TY>), dim3(grid),
+      hipLaunchKernelGGL((k_fused_dma<S, ES, T, SHIFT, COLL, TX, TY, false, XSH>), dim3(grid),
                          dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H, ntx, nty,
-                         ntiles);
+                         ntiles, nullptr, nullptr, T(0), T(0), T(0), XSH ? s->xsh : 0);
       return hipGetLastError();
     } else {
       return hipErrorInvalidConfiguration;
@@ -139,11 +139,13 @@ struct Impl {
   // 4 re-read with prefetch and >= 3 waves/SIMD; 5 and 6 are 1 and 0 on the
   // pull form (run() converts to and from it around the steps); 7 and 8
   // are 0 and 6 with two lead planes per barrier; 9 stages the lead plane in
-  // LDS by DMA (k_fused_dma)
+  // LDS by DMA (k_fused_dma); 10 is 9 with the populations kept in the
+  // x-shifted storage between the steps of a run
   template <int TX, int TY>
   static hipError_t variant(idmx_sc_s* s, const T* pop, T* swap) {
     switch (s->hist) {
       case 9: return launch_dma<TX, TY>(s, pop, swap);
+      case 10: return launch_dma<TX, TY, true>(s, pop, swap);
       case 7: return launch_fused<TX, TY, false, false, false, false, false, true>(
           s, pop, swap, nullptr, nullptr, nullptr);
       case 8: return launch_fused<TX, TY, false, false, false, false, true, true>(
@@ -296,6 +298,11 @@ struct Impl {
     // step in it, convert back (nsteps + 2 buffer swaps: same parity as
     // nsteps, so the caller's pointer bookkeeping is unchanged)
     const bool pull = COLL < 3 && !twop && !fwalls && pull_form(s->hist) && nsteps > 0;
+    // variant 10 keeps the populations in the x-shifted storage between its
+    // steps (k_fused_dma<..., XSH>): the first of the nxsh steps reads the
+    // streamed storage, the last writes it (the step that writes n/psi/u
+    // runs on the streamed storage), so a run of one step is variant 9
+    const int nxsh = COLL < 3 && !twop && !fwalls && s->hist == 10 ? nsteps - (wf ? 1 : 0) : 0;
     if constexpr (COLL < 3) if (pull) {
       hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
                          (const T*)*pop, (T*)*swap, s->geo, 1);
@@ -328,8 +335,10 @@ struct Impl {
           e = fused_walls(s, a, b, psi);
         } else {
           if (mid) IDMX_HIP(hipEventRecord(mid, st));
+          s->xsh = k < nxsh ? (k > 0 ? 1 : 0) | (k < nxsh - 1 ? 2 : 0) : 0;
           if (wf && last) e = fused<true>(s, a, b, n, psi, u);
           else e = fused<false>(s, a, b, nullptr, nullptr, nullptr);
+          s->xsh = 0;
         }
       }
       if (e != hipSuccess) return hip_fail(e, "idmx_sc_run launch");
@@ -454,9 +463,9 @@ struct Impl {
   }
 
   // Whether launch variant v can run the tile tx x ty (the LDS staging of
-  // variant 9 has to fit one CU).
+  // variants 9 and 10 has to fit one CU).
   static int launch_ok(int tx, int ty, int v) {
-    if (v != 9) return 1;
+    if (v != 9 && v != 10) return 1;
     if constexpr (S::D == 2) {
       switch (tx) {
         case 64: return DmaShape<S, ES, T, 64, 1, COLL>::fits;

@@ -978,12 +978,13 @@ __device__ __forceinline__ void dma_dword(uint64_t base, uint32_t voff, uint32_t
 // body force (bx, by, bz) added to the Shan-Chen force: psi_w is the psi
 // array (its non-fluid sites are staged with the populations), wmask the
 // per-site masks (kWallSelf). FP64 only.
-template <class S, class ES, class T, bool SHIFT, int COLL, int TX, int TY, bool WALLS = false>
+template <class S, class ES, class T, bool SHIFT, int COLL, int TX, int TY, bool WALLS = false,
+          bool XSH = false>
 __global__ void __launch_bounds__(TX * TY)
 k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<T> P, int H,
             int ntx, int nty, int ntiles, const double* __restrict__ psi_w = nullptr,
             const uint64_t* __restrict__ wmask = nullptr, T bx = T(0), T by = T(0),
-            T bz = T(0)) {
+            T bz = T(0), int xsh = 0) {
   using SH = DmaShape<S, ES, T, TX, TY, COLL, WALLS>;
   constexpr int D = S::D, Q = S::Q, R = ES::R;
   constexpr int NT = SH::NT, PX = SH::PX, PY = SH::PY, NP = SH::NP, NS = SH::NS;
@@ -1030,6 +1031,16 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
   // Per-lane source byte offsets (within a plane of one population) of the
   // chunks this wave fetches, and whether the lane's dword exists.
   uint32_t soff[KW];
+  // XSH: the x-shifted storage, in which population q of site x is kept at
+  // x - c_qx. Stores into it cover the tile's own, line-aligned x range for
+  // every population (streamed storage: x + c_qx, one site off the lines for
+  // the populations with c_qx != 0); copies from it read the populations
+  // with c_qx = +1 / -1 from x - 1 / x + 1. xsh bit 0: the step reads the
+  // shifted storage, bit 1: it writes it (a run converts on its first and
+  // last step, with no pass of its own).
+  static_assert(!(XSH && WALLS), "the x-shifted storage is not used with walls");
+  const bool rsh = XSH && (xsh & 1), wsh = XSH && (xsh & 2);
+  uint32_t soffp[XSH ? KW : 1], soffm[XSH ? KW : 1];
   bool sval[KW];
 #pragma unroll
   for (int kk = 0; kk < KW; kk++) {
@@ -1042,6 +1053,15 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
     const int gy = D == 3 ? ((y0 - R + hy) % geo.L1 + geo.L1) % geo.L1 : 0;
     soff[kk] = ((uint32_t)gx + (uint32_t)geo.L0 * (uint32_t)gy) * (uint32_t)sizeof(T) +
                (uint32_t)(d % EPD) * 4u;
+    if constexpr (XSH) {
+      // the streamed storage's offsets where this step reads it
+      const uint32_t gxp = rsh ? (uint32_t)wrap(gx - 1, geo.L0) : (uint32_t)gx;
+      const uint32_t gxm = rsh ? (uint32_t)wrap(gx + 1, geo.L0) : (uint32_t)gx;
+      soffp[kk] = (gxp + (uint32_t)geo.L0 * (uint32_t)gy) * (uint32_t)sizeof(T) +
+                  (uint32_t)(d % EPD) * 4u;
+      soffm[kk] = (gxm + (uint32_t)geo.L0 * (uint32_t)gy) * (uint32_t)sizeof(T) +
+                  (uint32_t)(d % EPD) * 4u;
+    }
   }
   const uint64_t qstride = (uint64_t)geo.V * sizeof(T);
   const uint32_t stage_lds = __builtin_amdgcn_readfirstlane(
@@ -1059,7 +1079,11 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
       for (int kk = 0; kk < KW; kk++) {
         const int k = wv + kk * NW;
         if (k < NK && sval[kk])
-          dma_dword(base, soff[kk], stage_lds + (uint32_t)(q * SH::QB + k * 256));
+          dma_dword(base,
+                    !XSH || S::cx[q] == 0 ? soff[kk]
+                                          : S::cx[q] > 0 ? soffp[kk % (XSH ? KW : 1)]
+                                                         : soffm[kk % (XSH ? KW : 1)],
+                    stage_lds + (uint32_t)(q * SH::QB + k * 256));
       }
     }
     if constexpr (WALLS) {
@@ -1106,7 +1130,9 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       const int xx = a == 0 ? xm : a == 1 ? x : xp;
-      toff[b][a] = ((uint32_t)xx + (uint32_t)geo.L0 * (uint32_t)(D == 3 ? yy : 0)) *
+      // where this step writes the x-shifted storage every population of
+      // the site goes to its own x
+      toff[b][a] = ((uint32_t)(wsh ? x : xx) + (uint32_t)geo.L0 * (uint32_t)(D == 3 ? yy : 0)) *
                    (uint32_t)sizeof(T);
     }
   }

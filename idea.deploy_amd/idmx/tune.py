@@ -36,7 +36,7 @@ def sim_candidates(D, L_march):
     tiles = ((64, 1), (128, 1), (256, 1)) if D == 2 else ((32, 8), (64, 4), (128, 2), (256, 2))
     Hs = [h for h in (32, 64, 128) if h <= L_march] or [L_march]
     return [(tx, ty, h, var) for tx, ty in tiles for h in Hs
-            for var in (1, 0, 2, 3, 4, 5, 6, 7, 8, 9)]
+            for var in (1, 0, 2, 3, 4, 5, 6, 7, 8, 9, 10)]
 
 
 def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
