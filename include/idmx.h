@@ -298,7 +298,12 @@ int idmx_sc_pop_from_f64(idmx_sc_t sim, void* pop, const double* src_dev);
  * barrier, 9 holds them in registers and copies each lead plane (tile and
  * halo) into LDS by DMA while the previous plane collides (refused with
  * IDMX_E_ARG for tiles whose staging buffer exceeds the 160 KiB of LDS of a
- * CU, e.g. FP64 256x2). Chosen at creation;
+ * CU, e.g. FP64 256x2), 10 is 9 with the populations kept, between the
+ * steps of one idmx_sc_run, in an x-shifted storage (population q of site x
+ * at x - c_qx, so every push store is line-aligned; the run's first step
+ * reads the streamed storage and its last writes it, so pop holds streamed
+ * populations between calls, and the results equal variant 9's bit for
+ * bit). Chosen at creation;
  * the autotuner (idmx.tune) overrides it. No reference counterpart: the
  * reference fixes block = 128 threads (idpy/LBM/MultiPhase.py:2072-2085). */
 int idmx_sc_set_launch(idmx_sc_t sim, int tx, int ty, int H, int hist);
