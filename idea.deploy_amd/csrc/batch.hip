@@ -16,8 +16,8 @@ hipError_t idmx::batch_step_f64(idmx_batch_s* b, const void* in, void* out) {
   return step_es<double, false>(b, in, out);
 }
 
-hipError_t idmx::batch_two_steps_f64(idmx_batch_s* b, const void* in, void* out) {
-  return two_steps_tx<double, false>(b, in, out);
+hipError_t idmx::batch_two_steps_f64(idmx_batch_s* b, const void* in, void* out, int xsh) {
+  return two_steps_tx<double, false>(b, in, out, xsh);
 }
 
 namespace {
@@ -38,7 +38,7 @@ hipError_t restream(idmx_batch_s* b, int dir) {
 }
 
 // Tile shapes (tx x H) accepted by idmx_batch_set_launch for kinds 1 and 3,
-// and for kind 4 (two steps per pass).
+// and for kinds 4 and 5 (two steps per pass).
 bool tile_shape_ok(int tx, int H) {
   return (tx == 64 && H == 4) || (tx == 32 && H == 8) || (tx == 16 && H == 16) ||
          (tx == 32 && H == 4);
@@ -51,9 +51,9 @@ hipError_t one_step(idmx_batch_s* b, const void* in, void* out) {
   return b->prec == IDMX_PREC_F64 ? batch_step_f64(b, in, out) : batch_step_f32(b, in, out);
 }
 
-hipError_t two_steps(idmx_batch_s* b, const void* in, void* out) {
-  return b->prec == IDMX_PREC_F64 ? batch_two_steps_f64(b, in, out)
-                                  : batch_two_steps_f32(b, in, out);
+hipError_t two_steps(idmx_batch_s* b, const void* in, void* out, int xsh = 0) {
+  return b->prec == IDMX_PREC_F64 ? batch_two_steps_f64(b, in, out, xsh)
+                                  : batch_two_steps_f32(b, in, out, xsh);
 }
 
 template <class T, bool SHIFT>
@@ -208,15 +208,16 @@ int idmx_batch_destroy(idmx_batch_t b) {
 
 int idmx_batch_set_launch(idmx_batch_t b, int kind, int tx, int H, int grid) {
   BCHECK(b, "idmx_batch_set_launch: null batch");
-  BCHECK(kind >= 0 && kind <= 4,
+  BCHECK(kind >= 0 && kind <= 5,
          "idmx_batch_set_launch: kind is 0 (march), 1 (tile), 2 / 3 (pull-form march / tile), "
-         "4 (tile, two steps per pass)");
+         "4 (tile, two steps per pass), 5 (4 on the x-shifted storage)");
   if (kind == 0 || kind == 2) {
     BCHECK(tx == 64 || tx == 128 || tx == 256, "idmx_batch_set_launch: march tx is 64, 128 or 256");
     BCHECK(H >= 1, "idmx_batch_set_launch: H >= 1");
-  } else if (kind == 4) {
-    BCHECK(b->estencil == IDMX_E4, "idmx_batch_set_launch: kind 4 needs the E4 force");
-    BCHECK(tile2_shape_ok(tx, H), "idmx_batch_set_launch: kind 4 tiles are 32x8, 16x16, 32x16");
+  } else if (kind == 4 || kind == 5) {
+    BCHECK(b->estencil == IDMX_E4, "idmx_batch_set_launch: kinds 4 and 5 need the E4 force");
+    BCHECK(tile2_shape_ok(tx, H),
+           "idmx_batch_set_launch: kind 4 and 5 tiles are 32x8, 16x16, 32x16");
   } else {
     BCHECK(tile_shape_ok(tx, H),
            "idmx_batch_set_launch: tile is 64x4, 32x8, 16x16 or 32x4");
@@ -297,9 +298,12 @@ int idmx_batch_run(idmx_batch_t b, int nsteps) {
   // state in the same buffer) is captured before the timed region, for the
   // buffer parity the steps will see: flipped by the conversion to the
   // pull form.
-  // kind 4 advances two steps per launch, so its graph holds an even number
-  // of launches (a multiple of 4 steps) to end in the buffer it started in
-  const bool pairs = b->kind == 4;
+  // kinds 4 and 5 advance two steps per launch, so their graph holds an even
+  // number of launches (a multiple of 4 steps) to end in the buffer it
+  // started in. Kind 5 keeps the populations in the x-shifted storage between
+  // the graph's launches: the first reads the streamed storage, the last
+  // writes it, so every replay starts and ends streamed.
+  const bool pairs = b->kind == 4 || b->kind == 5;
   const int G = pairs ? 4 * std::max(1, std::min(nsteps / 4, 25))
                       : 2 * std::max(1, std::min(nsteps / 2, 50));
   const int cur_steps = b->cur ^ (pull ? 1 : 0);
@@ -307,9 +311,11 @@ int idmx_batch_run(idmx_batch_t b, int nsteps) {
     drop_graph(b);
     hipGraph_t g;
     IDMX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < (pairs ? G / 2 : G); k++) {
+    const int nl = pairs ? G / 2 : G;
+    for (int k = 0; k < nl; k++) {
       const int src = (cur_steps + k) & 1;
-      hipError_t e = pairs ? two_steps(b, b->pop[src], b->pop[src ^ 1])
+      const int xsh = b->kind == 5 ? (k > 0 ? 1 : 0) | (k < nl - 1 ? 2 : 0) : 0;
+      hipError_t e = pairs ? two_steps(b, b->pop[src], b->pop[src ^ 1], xsh)
                            : one_step(b, b->pop[src], b->pop[src ^ 1]);
       if (e != hipSuccess) {
         hipStreamEndCapture(st, &g);

@@ -9,8 +9,8 @@ hipError_t batch_step_f32(idmx_batch_s* b, const void* in, void* out) {
   return step_es<float, true>(b, in, out);
 }
 
-hipError_t batch_two_steps_f32(idmx_batch_s* b, const void* in, void* out) {
-  return two_steps_tx<float, true>(b, in, out);
+hipError_t batch_two_steps_f32(idmx_batch_s* b, const void* in, void* out, int xsh) {
+  return two_steps_tx<float, true>(b, in, out, xsh);
 }
 
 hipError_t batch_init_f32(idmx_batch_s* b, float* pop, const double* n, int64_t V, double rho0,

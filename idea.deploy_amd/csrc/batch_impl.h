@@ -79,7 +79,7 @@ hipError_t launch_tile(idmx_batch_s* b, const void* in, void* out) {
 
 template <class ES, class T, bool SHIFT>
 hipError_t step_tx(idmx_batch_s* b, const void* in, void* out) {
-  if (b->kind == 1 || b->kind == 3 || b->kind == 4) {
+  if (b->kind == 1 || b->kind == 3 || b->kind == 4 || b->kind == 5) {
     if (b->tx == 64) return launch_tile<ES, T, SHIFT, 64, 4>(b, in, out);
     if (b->tx == 16) return launch_tile<ES, T, SHIFT, 16, 16>(b, in, out);
     if (b->H == 4) return launch_tile<ES, T, SHIFT, 32, 4>(b, in, out);
@@ -102,27 +102,29 @@ hipError_t step_es(idmx_batch_s* b, const void* in, void* out) {
   }
 }
 
-// Two steps of every live domain in one launch (kind 4, E4 tiles).
+// Two steps of every live domain in one launch (kinds 4 and 5, E4 tiles).
 template <class T, bool SHIFT, int BX, int BY>
-hipError_t launch_tile2(idmx_batch_s* b, const void* in, void* out) {
+hipError_t launch_tile2(idmx_batch_s* b, const void* in, void* out, int xsh) {
   hipLaunchKernelGGL((k_batched_tile2<T, SHIFT, BX, BY>), dim3(step_grid(b)), dim3(BX * BY), 0,
-                     b->ctx->stream, (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems);
+                     b->ctx->stream, (const T*)in, (T*)out, b->d_doms, b->d_items, b->d_nitems,
+                     xsh);
   return hipGetLastError();
 }
 
 template <class T, bool SHIFT>
-hipError_t two_steps_tx(idmx_batch_s* b, const void* in, void* out) {
-  if (b->tx == 16) return launch_tile2<T, SHIFT, 16, 16>(b, in, out);
-  if (b->H == 16) return launch_tile2<T, SHIFT, 32, 16>(b, in, out);
-  return launch_tile2<T, SHIFT, 32, 8>(b, in, out);
+hipError_t two_steps_tx(idmx_batch_s* b, const void* in, void* out, int xsh) {
+  if (b->tx == 16) return launch_tile2<T, SHIFT, 16, 16>(b, in, out, xsh);
+  if (b->H == 16) return launch_tile2<T, SHIFT, 32, 16>(b, in, out, xsh);
+  return launch_tile2<T, SHIFT, 32, 8>(b, in, out, xsh);
 }
 
 // One step of every live domain, in the FP64 and the FP32-shifted precision,
-// and two steps in one launch (kind 4).
+// and two steps in one launch (kinds 4 and 5; xsh: the x-shifted storage
+// flags of k_batched_tile2).
 hipError_t batch_step_f64(idmx_batch_s* b, const void* in, void* out);
 hipError_t batch_step_f32(idmx_batch_s* b, const void* in, void* out);
-hipError_t batch_two_steps_f64(idmx_batch_s* b, const void* in, void* out);
-hipError_t batch_two_steps_f32(idmx_batch_s* b, const void* in, void* out);
+hipError_t batch_two_steps_f64(idmx_batch_s* b, const void* in, void* out, int xsh);
+hipError_t batch_two_steps_f32(idmx_batch_s* b, const void* in, void* out, int xsh);
 // FP32-shifted populations of one domain at rest equilibrium of density n.
 hipError_t batch_init_f32(idmx_batch_s* b, float* pop, const double* n, int64_t V, double rho0,
                           unsigned grid);

@@ -248,6 +248,10 @@ k_batched_tile(const T* __restrict__ pop_all, T* __restrict__ swap_all,
 //   phase C: f(1)(y) = f*(0)(y - c_q) on the tile plus a 1-site ring, n(1),
 //            psi(1) -> LDS;
 //   phase D: collide the tile's own sites and push-store f*(1).
+// xsh (kind 5): bit 0, phase A reads the x-shifted storage (population q of
+// site x kept at x - c_qx); bit 1, phase D writes it, so that the stores of
+// every population cover the tile's own x range. idmx_batch_run's graph
+// enters and leaves it within each replay.
 template <int BX, int BY>
 struct Tile2Shape {
   static constexpr int NT = BX * BY;
@@ -262,7 +266,8 @@ struct Tile2Shape {
 template <class T, int BX, int BY>
 __device__ __forceinline__ void tile2_load(const T* __restrict__ pop, const BatchItem& w,
                                            const BatchDom& d,
-                                           T (&f0)[Tile2Shape<BX, BY>::K][D2Q9::Q]) {
+                                           T (&f0)[Tile2Shape<BX, BY>::K][D2Q9::Q], bool rsh) {
+  using S = D2Q9;
   using SH = Tile2Shape<BX, BY>;
   const int64_t V = (int64_t)d.L0 * d.L1;
 #pragma unroll
@@ -272,8 +277,12 @@ __device__ __forceinline__ void tile2_load(const T* __restrict__ pop, const Batc
       const int lx = sidx % SH::LW, ly = sidx / SH::LW;
       const int xs = pmod(w.x0 - 3 + lx, d.L0), ys = pmod(w.m0 - 3 + ly, d.L1);
       const int64_t g = (int64_t)ys * d.L0 + xs;
+      // the x-shifted storage keeps population q of site x at x - c_qx
+      const int64_t gp = rsh ? (int64_t)ys * d.L0 + pmod(xs - 1, d.L0) : g;
+      const int64_t gm = rsh ? (int64_t)ys * d.L0 + pmod(xs + 1, d.L0) : g;
 #pragma unroll
-      for (int q = 0; q < D2Q9::Q; q++) f0[k][q] = pop[(size_t)q * V + g];
+      for (int q = 0; q < S::Q; q++)
+        f0[k][q] = pop[(size_t)q * V + (S::cx[q] > 0 ? gp : S::cx[q] < 0 ? gm : g)];
     }
   }
 }
@@ -285,7 +294,7 @@ template <class T, bool SHIFT, int BX, int BY>
 __device__ __forceinline__ void tile2_body(
     const T (&f0)[Tile2Shape<BX, BY>::K][D2Q9::Q], const BatchItem& w, const BatchDom& d,
     T* __restrict__ swap_all, T (*a0)[Tile2Shape<BX, BY>::NC],
-    T (*psi0)[Tile2Shape<BX, BY>::LW], T (*psi1)[Tile2Shape<BX, BY>::PW]) {
+    T (*psi0)[Tile2Shape<BX, BY>::LW], T (*psi1)[Tile2Shape<BX, BY>::PW], bool wsh) {
   using S = D2Q9;
   using ES = D2E4;
   using SH = Tile2Shape<BX, BY>;
@@ -361,7 +370,16 @@ __device__ __forceinline__ void tile2_body(
     }
     T fo[Q], u[3];
     collide<S, T, SHIFT, 0>(f, s, gx, gy, T(0), psi1[ty + 1][tx + 1], P, fo, u);
-    push_store<S, T>(swap_all + d.pop_off, geo, x, y, 0, fo);
+    if (wsh) {
+      // x-shifted storage: f*_q of site x goes to x + c_q, kept at its x
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const int yt = S::cy[q] ? wrap(y + S::cy[q], geo.L1) : y;
+        pop_st_nt(swap_all + d.pop_off, q, geo.V, (int64_t)yt * d.L0 + x, fo[q]);
+      }
+    } else {
+      push_store<S, T>(swap_all + d.pop_off, geo, x, y, 0, fo);
+    }
   }
   lds_barrier();
 }
@@ -370,7 +388,7 @@ template <class T, bool SHIFT, int BX, int BY>
 __global__ void __launch_bounds__(BX * BY)
 k_batched_tile2(const T* __restrict__ pop_all, T* __restrict__ swap_all,
                 const BatchDom* __restrict__ doms, const BatchItem* __restrict__ items,
-                const int* __restrict__ n_items) {
+                const int* __restrict__ n_items, int xsh) {
   using SH = Tile2Shape<BX, BY>;
   __shared__ T a0[D2Q9::Q][SH::NC];      // f*(0) on C0
   __shared__ T psi0[SH::LH][SH::LW];
@@ -380,8 +398,8 @@ k_batched_tile2(const T* __restrict__ pop_all, T* __restrict__ swap_all,
     const BatchItem w = items[it];
     const BatchDom d = doms[w.dom];
     T f0[SH::K][D2Q9::Q];
-    tile2_load<T, BX, BY>(pop_all + d.pop_off, w, d, f0);
-    tile2_body<T, SHIFT, BX, BY>(f0, w, d, swap_all, a0, psi0, psi1);
+    tile2_load<T, BX, BY>(pop_all + d.pop_off, w, d, f0, (xsh & 1) != 0);
+    tile2_body<T, SHIFT, BX, BY>(f0, w, d, swap_all, a0, psi0, psi1, (xsh & 2) != 0);
   }
 }
 

@@ -128,7 +128,8 @@ class Batch:
     def set_launch(self, kind, tx, H, grid=0):
         """kind 0: marching items of tx x H rows; kind 1: tx x H tiles;
         kinds 2 / 3: 0 / 1 on the pull form; kind 4: tiles advanced two
-        steps per launch (E4)."""
+        steps per launch (E4); kind 5: kind 4 on the x-shifted storage
+        between the launches of a replayed graph (line-aligned stores)."""
         _lib.call('idmx_batch_set_launch', self.handle, int(kind), int(tx), int(H), int(grid))
 
     def get_launch(self):

@@ -86,11 +86,12 @@ def tune_sim(engine, n0=None, steps=10, use_cache=True, apply=True):
 
 def batch_shapes(estencil='E4'):
     """(kind, tx, H): marching items and tiles, push and pull form, and with
-    the E4 force tiles advanced two steps per launch (kind 4)."""
+    the E4 force tiles advanced two steps per launch (kind 4), also on the
+    x-shifted storage (kind 5)."""
     shapes = [(k, tx, H) for k in (0, 2) for tx in (64, 128, 256) for H in (8, 16, 32)]
     shapes += [(k, tx, H) for k in (1, 3) for tx, H in ((64, 4), (32, 8), (16, 16), (32, 4))]
     if estencil == 'E4':
-        shapes += [(4, tx, H) for tx, H in ((32, 8), (16, 16), (32, 16))]
+        shapes += [(k, tx, H) for k in (4, 5) for tx, H in ((32, 8), (16, 16), (32, 16))]
     return shapes
 
 

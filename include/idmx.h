@@ -376,7 +376,12 @@ int idmx_batch_destroy(idmx_batch_t b);
  * tiles (32x8, 16x16, 32x16) advanced two steps per launch, the first
  * step's populations kept in LDS for the tile plus a 2-site ring (half the
  * HBM bytes and half the launches per step; an odd step left over runs as
- * kind 1); the same results as kind 1, bit for bit. grid = blocks of a step
+ * kind 1); the same results as kind 1, bit for bit. kind 5: kind 4 with
+ * the populations kept, between the launches of idmx_batch_run's captured
+ * graph, in an x-shifted storage (population q of site x at x - c_qx, so
+ * every push store covers the tile's own x range); each replay reads and
+ * writes the streamed storage at its ends, and the results equal kind 4's
+ * bit for bit. grid = blocks of a step
  * launch (0 keeps the current grid); a step launches at most one block per
  * live work item, and each XCD takes a contiguous run of items. Picked by
  * the autotuner (idmx.tune). */

@@ -39,19 +39,20 @@ def test_laplace_probe_sign_and_value():
 def test_batch_candidates_grids():
     """Every shape with 4, 8 and 16 blocks per CU, plus one block per work
     item where the batch has more items than the largest of those grids;
-    the two-steps-per-launch tiles (kind 4) only with the E4 force."""
+    the two-steps-per-launch tiles (kinds 4 and 5) only with the E4 force."""
     from idmx import tune
     shapes = tune.batch_shapes()
-    assert len(set(shapes)) == len(shapes) == 29
-    assert sorted(s for s in shapes if s[0] == 4) == [(4, 16, 16), (4, 32, 8), (4, 32, 16)]
-    assert not [s for s in tune.batch_shapes('E8') if s[0] == 4]
+    assert len(set(shapes)) == len(shapes) == 32
+    for k in (4, 5):
+        assert sorted(s for s in shapes if s[0] == k) == [(k, 16, 16), (k, 32, 8), (k, 32, 16)]
+    assert not [s for s in tune.batch_shapes('E8') if s[0] in (4, 5)]
     base = tune.batch_candidates(256)
     assert len(base) == 3 * len(shapes)
     assert {c[3] for c in base} == {1024, 2048, 4096}
-    items = {s: (5000 if s[0] in (1, 3, 4) else 4096) for s in shapes}
+    items = {s: (5000 if s[0] in (1, 3, 4, 5) else 4096) for s in shapes}
     cands = tune.batch_candidates(256, items)
     extra = [c for c in cands if c not in base]
-    assert sorted(extra) == sorted(s + (5000,) for s in shapes if s[0] in (1, 3, 4))
+    assert sorted(extra) == sorted(s + (5000,) for s in shapes if s[0] in (1, 3, 4, 5))
 
 
 def test_droplet_centre_offsets():
