@@ -177,17 +177,30 @@ struct Impl {
       if constexpr (S::D == 2) return launch_fused<64, 1, true, true>(s, pop, swap, n, psi, u);
       else return launch_fused<32, 8, true, true>(s, pop, swap, n, psi, u);
     } else {
+      // the LDS-staged step on the x-shifted storage (s->xsh) where it
+      // applies, else the populations held in registers
       if constexpr (S::D == 2) {
-        if (s->geo.L0 >= 256)
+        if (s->geo.L0 >= 256) {
+          if constexpr (kFixedDma2) return launch_dma<256, 1, true>(s, pop, swap);
           return launch_fused<256, 1, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
+        }
+        if constexpr (kFixedDma2) return launch_dma<64, 1, true>(s, pop, swap);
         return launch_fused<64, 1, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
       } else {
+        if constexpr (kFixedDma3) return launch_dma<64, 4, true>(s, pop, swap);
         if (s->geo.L0 >= 256)
           return launch_fused<256, 2, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
         return launch_fused<64, 4, false, true>(s, pop, swap, nullptr, nullptr, nullptr);
       }
     }
   }
+  // whether fused_fixed<false> runs the LDS-staged step: 2D rows of 256 and
+  // 64 sites, 3D 64x4 tiles with the reach-1 stencil (a reach-2 staging
+  // leaves one workgroup per CU)
+  static constexpr bool kFixedDma2 = S::D == 2 && DmaShape<S, ES, T, 256, 1, COLL>::fits &&
+                                     DmaShape<S, ES, T, 64, 1, COLL>::fits;
+  static constexpr bool kFixedDma3 =
+      S::D == 3 && ES::R == 1 && DmaShape<S, ES, T, 64, 4, COLL>::fits;
 
   // Tile shapes the autotuner chooses from; the step that writes n/psi/u at
   // a checkpoint uses one fixed shape.
@@ -302,7 +315,8 @@ struct Impl {
     // steps (k_fused_dma<..., XSH>): the first of the nxsh steps reads the
     // streamed storage, the last writes it (the step that writes n/psi/u
     // runs on the streamed storage), so a run of one step is variant 9
-    const int nxsh = COLL < 3 && !twop && !fwalls && s->hist == 10 ? nsteps - (wf ? 1 : 0) : 0;
+    const bool xs = COLL < 3 ? s->hist == 10 : (kFixedDma2 || kFixedDma3);
+    const int nxsh = xs && !twop && !fwalls ? nsteps - (wf ? 1 : 0) : 0;
     if constexpr (COLL < 3) if (pull) {
       hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
                          (const T*)*pop, (T*)*swap, s->geo, 1);
@@ -324,8 +338,10 @@ struct Impl {
           e = two_pass(s, a, b, n, psi, u, wf && last, mid);
         } else {
           if (mid) IDMX_HIP(hipEventRecord(mid, st));
+          s->xsh = k < nxsh ? (k > 0 ? 1 : 0) | (k < nxsh - 1 ? 2 : 0) : 0;
           if (wf && last) e = fused_fixed<true>(s, a, b, n, psi, u);
           else e = fused_fixed<false>(s, a, b, nullptr, nullptr, nullptr);
+          s->xsh = 0;
         }
       } else {
         if (twop || (fwalls && wf && last)) {

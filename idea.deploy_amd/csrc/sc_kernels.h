@@ -1017,7 +1017,9 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
   // collision tables in LDS: read there with LDS loads, not with vector
   // memory loads, whose completion would have to wait for the plane copy
   Phys<T> PL = P;
-  static_assert(COLL >= 0 && COLL <= 2, "the LDS-staged step runs BGK and MRT with Guo forcing");
+  static_assert(COLL >= 0 && COLL <= 4 && !(WALLS && COLL > 2),
+                "the LDS-staged step runs BGK and MRT with the Guo forcing, BGK with the "
+                "Shan-Chen forcing and BGK with the isotropy filter");
   if constexpr (COLL == 1 || COLL == 2) {
     for (int i = tid; i < SH::NTAB; i += NT) tab[i] = P.mrtA[i];
     PL.mrtA = tab;

@@ -65,8 +65,9 @@ enum {
  * every site, by the isotropy filter of the post-collision populations
  * (K_IsotropyFilter, idpy/LBM/LBMKernelsMeta.py:895-1080, used by
  * MainLoopIsoFilter, idpy/LBM/MultiPhase.py:1099-1223). Both run on one
- * fixed fused launch (not autotuned), and on the two-pass step with walls,
- * a body force, a psi program or IDMX_RUN_TWO_PASS. */
+ * fixed fused launch (not autotuned: the LDS-staged step on the x-shifted
+ * storage, as launch variant 10, except 3D E6/E8), and on the two-pass step
+ * with walls, a body force, a psi program or IDMX_RUN_TWO_PASS. */
 enum { IDMX_BGK = 0, IDMX_MRT = 1, IDMX_BGK_SCF = 2, IDMX_BGK_ISO = 3 };
 enum { IDMX_PREC_F64 = 0, IDMX_PREC_F32S = 1 };
 
