@@ -2,7 +2,9 @@
 accepts (2 R + 1, so the halo of a tile wraps onto the tile itself), and
 ragged boxes with one axis at that size and another long, against the
 oracle (FP64, 1e-12): the default fused launch (writing n / psi on the last
-step), the two-pass step, and in 3D the LDS-staged step (64 x 4 tiles)."""
+step), the two-pass step, and the LDS-staged step (3D 64 x 4 tiles on the
+streamed and on the x-shifted storage, 2D rows of 64 on the x-shifted
+storage)."""
 import numpy as np
 import pytest
 
@@ -35,7 +37,9 @@ def test_smallest_and_ragged_lattices_match_oracle(tenet, xi, es, dims):
     cpuref.meta_run(st, es, G, 1.0, nsteps=nsteps)
     modes = [('fused', None), ('two_pass', None)]
     if xi == 'D3Q19':
-        modes.append(('fused', (64, 4, 8, 9)))
+        modes += [('fused', (64, 4, 8, 9)), ('fused', (64, 4, 8, 10))]
+    else:
+        modes.append(('fused', (64, 1, 8, 10)))
     for mode, launch in modes:
         eng = SCEngine(tenet, xi, es, dims, 'exp', G, 1.0)
         if launch is not None:

@@ -122,7 +122,8 @@ def test_c1_droplet_full_length_meta_class(tenet):
 
 def test_d3q19_mrt_lds_staged_long_run_matches_oracle(tenet):
     """The benchmark's kernel (lead plane copied to LDS by DMA, 64x4 tiles,
-    explicit vmcnt waits) over 600 steps of a D3Q19 MRT droplet on a box that
+    explicit vmcnt waits, x-shifted storage between the steps) over 600
+    steps of a D3Q19 MRT droplet on a box that
     leaves partial tiles and a short last march segment, against the
     oracle: a copy that landed late or a stale staging read would drift.
     Measured 7.0e-13: the GPU applies the diagonal MRT in moment space with
@@ -139,7 +140,7 @@ def test_d3q19_mrt_lds_staged_long_run_matches_oracle(tenet):
     s = rng.uniform(0.7, 1.5, size=19)
     eng = SCEngine(tenet, 'D3Q19', 'D3E4', dims, 'exp', G, 1.0,
                    mrt=hermite.mrt_kernel_tables('D3Q19', np.diag(s)))
-    eng.set_launch(64, 4, 24, 9)
+    eng.set_launch(64, 4, 24, 10)
     n0 = droplet_field(dims, seed=31)
     f, m0, m1 = _run_engine(tenet, eng, n0, 600)
     eng.close()

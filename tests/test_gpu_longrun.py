@@ -1,7 +1,8 @@
 """Long runs of the benchmark physics on reduced lattices, every field against
 the oracle at 1e-12: D2Q9 with the E8 force and several droplets over 2000
-steps, and the D3Q19 MRT droplet over 5000 steps on the LDS-staged fused
-step (64x4 tiles, partial tiles and a short last segment), compared with the
+steps (rows of 256 on the x-shifted storage), and the D3Q19 MRT droplet over
+5000 steps on the LDS-staged fused step on the x-shifted storage (64x4
+tiles, partial tiles and a short last segment), compared with the
 oracle's moment-space evaluation of the same collision. The lattices are
 sized so the serial oracle finishes within about a minute."""
 import numpy as np
@@ -53,7 +54,7 @@ def test_d2q9_e8_droplets_2000_steps(tenet):
     dims = (200, 200)
     n0 = configs.droplets2d(L=200, count=5, rmin=10.0, rmax=28.0)
     eng = SCEngine(tenet, 'D2Q9', 'D2E8', dims, 'exp', G, 1.0)
-    eng.set_launch(256, 1, 32, 5)   # the benchmark's launch for this physics
+    eng.set_launch(256, 1, 32, 10)   # the benchmark's launch for this physics
     f, (n, psi, u), m0, m1 = _run(tenet, eng, n0, 2000, checkpoints=4)
     eng.close()
     st = cpuref.State('D2Q9', dims)
@@ -81,7 +82,7 @@ def test_d3q19_mrt_droplet_5000_steps(tenet, rates):
     s = np.ones(19) if rates == 'unit' else np.random.default_rng(17).uniform(0.7, 1.5, 19)
     eng = SCEngine(tenet, 'D3Q19', 'D3E4', dims, 'exp', G, 1.0,
                    mrt=hermite.mrt_kernel_tables('D3Q19', np.diag(s)))
-    eng.set_launch(64, 4, 16, 9)    # the benchmark's kernel
+    eng.set_launch(64, 4, 16, 10)    # the benchmark's kernel
     f, (n, psi, u), m0, m1 = _run(tenet, eng, n0, 5000, checkpoints=5)
     eng.close()
     st = cpuref.State('D3Q19', dims)
