@@ -95,6 +95,7 @@ struct Impl {
   // on narrow lattices); FP64 BGK only (kFusedWalls).
   template <int TX, int TY>
   static hipError_t launch_dma_walls(idmx_sc_s* s, const T* pop, T* swap, const double* psi) {
+    // on the x-shifted storage between the steps of a run (s->xsh)
     if constexpr (DmaShape<S, ES, T, TX, TY, COLL, true>::fits && !SHIFT && COLL == 0) {
       const Geom& g = s->geo;
       const int ntx = (g.L0 + TX - 1) / TX;
@@ -104,10 +105,10 @@ struct Impl {
       const int nseg = (Lm + H - 1) / H;
       const int ntiles = ntx * nty * nseg;
       const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
-      hipLaunchKernelGGL((k_fused_dma<S, ES, T, SHIFT, COLL, TX, TY, true>), dim3(grid),
+      hipLaunchKernelGGL((k_fused_dma<S, ES, T, SHIFT, COLL, TX, TY, true, true>), dim3(grid),
                          dim3(TX * TY), 0, s->ctx->stream, pop, swap, g, phys(s), H, ntx, nty,
                          ntiles, psi, (const uint64_t*)s->wmask_dev, (T)s->bforce[0],
-                         (T)s->bforce[1], (T)s->bforce[2]);
+                         (T)s->bforce[1], (T)s->bforce[2], s->xsh);
       return hipGetLastError();
     } else {
       return hipErrorInvalidConfiguration;
@@ -264,8 +265,8 @@ struct Impl {
     // walls (with or without a body force) run the fused step with walls; the
     // step that writes n / psi / u at the end of a run is the two-pass one
     // (from 2^19 sites up, or when asked for: on smaller lattices the
-    // two-pass step is as fast; measured 0.94x at 64^3, 1.09x at 1024^2,
-    // 1.20-1.36x at 128^3, 256^3 and 4096^2)
+    // two-pass step is as fast; measured 0.94x at 64^3, 1.07x at 1024^2,
+    // 1.30x at 256^3 and 1.47x at 4096^2)
     const bool fwalls = kFusedWalls && s->walls && (flags & IDMX_RUN_TWO_PASS) == 0 &&
                         s->desc.psi_kind != IDMX_PSI_PROGRAM &&
                         (s->geo.V >= (int64_t(1) << 19) || (flags & IDMX_RUN_FUSED_WALLS) != 0);
@@ -315,8 +316,8 @@ struct Impl {
     // steps (k_fused_dma<..., XSH>): the first of the nxsh steps reads the
     // streamed storage, the last writes it (the step that writes n/psi/u
     // runs on the streamed storage), so a run of one step is variant 9
-    const bool xs = COLL < 3 ? s->hist == 10 : (kFixedDma2 || kFixedDma3);
-    const int nxsh = xs && !twop && !fwalls ? nsteps - (wf ? 1 : 0) : 0;
+    const bool xs = fwalls || (COLL < 3 ? s->hist == 10 : (kFixedDma2 || kFixedDma3));
+    const int nxsh = xs && !twop ? nsteps - (wf ? 1 : 0) : 0;
     if constexpr (COLL < 3) if (pull) {
       hipLaunchKernelGGL((k_restream<S, T>), dim3(grid_for(s->geo.V)), dim3(256), 0, st,
                          (const T*)*pop, (T*)*swap, s->geo, 1);
@@ -348,7 +349,9 @@ struct Impl {
           e = two_pass(s, a, b, n, psi, u, wf && last, mid);
         } else if (fwalls) {
           if (mid) IDMX_HIP(hipEventRecord(mid, st));
+          s->xsh = k < nxsh ? (k > 0 ? 1 : 0) | (k < nxsh - 1 ? 2 : 0) : 0;
           e = fused_walls(s, a, b, psi);
+          s->xsh = 0;
         } else {
           if (mid) IDMX_HIP(hipEventRecord(mid, st));
           s->xsh = k < nxsh ? (k > 0 ? 1 : 0) | (k < nxsh - 1 ? 2 : 0) : 0;

@@ -1039,8 +1039,10 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
   // the populations with c_qx != 0); copies from it read the populations
   // with c_qx = +1 / -1 from x - 1 / x + 1. xsh bit 0: the step reads the
   // shifted storage, bit 1: it writes it (a run converts on its first and
-  // last step, with no pass of its own).
-  static_assert(!(XSH && WALLS), "the x-shifted storage is not used with walls");
+  // last step, with no pass of its own). With walls a population that
+  // bounces back goes to its own site as the opposite population, which
+  // that storage keeps at x + c_qx; the sites that are not fluid are never
+  // written, in either storage.
   const bool rsh = XSH && (xsh & 1), wsh = XSH && (xsh & 2);
   uint32_t soffp[XSH ? KW : 1], soffm[XSH ? KW : 1];
   bool sval[KW];
@@ -1133,8 +1135,10 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
     for (int a = 0; a < 3; a++) {
       const int xx = a == 0 ? xm : a == 1 ? x : xp;
       // where this step writes the x-shifted storage every population of
-      // the site goes to its own x
-      toff[b][a] = ((uint32_t)(wsh ? x : xx) + (uint32_t)geo.L0 * (uint32_t)(D == 3 ? yy : 0)) *
+      // the site goes to its own x (with walls the streamed offsets are
+      // kept: the bounce-back stores use them)
+      toff[b][a] = ((uint32_t)(wsh && !WALLS ? x : xx) +
+                    (uint32_t)geo.L0 * (uint32_t)(D == 3 ? yy : 0)) *
                    (uint32_t)sizeof(T);
     }
   }
@@ -1156,9 +1160,10 @@ k_fused_dma(const T* __restrict__ pop, T* __restrict__ pop_swap, Geom geo, Phys<
           const int dy = D == 3 ? S::cy[q] : 0;
           const uint64_t a_push = (uint64_t)(pop_swap) + (uint64_t)q * qbytes +
                                   (uint64_t)mt * (uint64_t)PS * sizeof(T) +
-                                  toff[dy + 1][S::cx[q] + 1];
+                                  toff[dy + 1][wsh ? 1 : S::cx[q] + 1];
           const uint64_t a_back = (uint64_t)(pop_swap) + (uint64_t)opposite<S>(q) * qbytes +
-                                  (uint64_t)pend_m * (uint64_t)PS * sizeof(T) + toff[1][1];
+                                  (uint64_t)pend_m * (uint64_t)PS * sizeof(T) +
+                                  toff[1][wsh ? S::cx[q] + 1 : 1];
           const bool back = q != 0 && ((pend_mask >> q) & 1);
           __builtin_nontemporal_store(fpend[q], (IDMX_GLOBAL T*)(back ? a_back : a_push));
         }

@@ -3,7 +3,10 @@ wall masks staged in LDS by DMA; fluid sites only, half-way bounce-back,
 optional body force) against the two-pass walls step on the same input:
 the same arithmetic in the same order, so the fluid sites agree to
 rounding. Solid blocks cross the periodic boundary and the tiles are
-partial."""
+partial. The fused steps of a run keep the populations in the x-shifted
+storage between them; a run that writes the fields ends with the two-pass
+step, one that does not with a fused step that writes the streamed storage
+(both are covered)."""
 import numpy as np
 import pytest
 
@@ -37,7 +40,8 @@ def _walls(dims, seed):
                                                  ('D2Q9', 'D2E4', (70, 33), 2e-5, 60),
                                                  ('D3Q19', 'D3E4', (70, 19, 13), 0.0, 30),
                                                  ('D3Q19', 'D3E4', (35, 17, 9), 1e-5, 30)])
-def test_fused_walls_step_equals_two_pass(tenet, xi, es, dims, g, nsteps):
+@pytest.mark.parametrize('wf', [True, False])
+def test_fused_walls_step_equals_two_pass(tenet, xi, es, dims, g, nsteps, wf):
     from idmx.IdpyCode import IdpyMemory
     from idmx.LBM._base import PopArray
     from idmx.LBM._engine import SCEngine
@@ -57,7 +61,7 @@ def test_fused_walls_step_equals_two_pass(tenet, xi, es, dims, g, nsteps):
         u = IdpyMemory.Zeros(D * V, np.float64, tenet)
         a, b = PopArray(eng, tenet), PopArray(eng, tenet)
         eng.init_pop(a, n, u)
-        if eng.run(a, b, nsteps, n=n, psi=psi, u=u, write_fields=True,
+        if eng.run(a, b, nsteps, n=n, psi=psi, u=u, write_fields=wf,
                    two_pass=mode == 'two_pass', fused_walls=mode == 'fused'):
             a, b = b, a
         ms = eng.last_run_ms()
@@ -73,6 +77,8 @@ def test_fused_walls_step_equals_two_pass(tenet, xi, es, dims, g, nsteps):
     print(xi, es, dims, 'g', g, 'pop', e_pop, 'identical', np.array_equal(f1[:, fl], f2[:, fl]),
           'ms per step fused', t1 / nsteps, 'two-pass', t2 / nsteps, flush=True)
     assert e_pop <= 1e-15
+    if not wf:
+        return
     assert rel_err(n1[fl], n2[fl]) <= 1e-15 and rel_err(p1[fl], p2[fl]) <= 1e-15
     assert rel_err(u1.reshape(D, V)[:, fl].ravel(), u2.reshape(D, V)[:, fl].ravel()) <= 1e-14
     assert np.array_equal(p1[~fl], psi_w[~fl])
