@@ -1,4 +1,4 @@
-"""Host logic of the sweep driver that needs no GPU: sweep point sampling,
+"""Host logic of the sweep driver and the autotuners that needs no GPU: sweep point sampling,
 the droplet end-state classifier, the Laplace pressure probe and the batch
 autotuner's candidate launches."""
 import numpy as np
@@ -68,3 +68,17 @@ def test_run_sweep_rejects_unknown_rest_init():
     from idmx import sweep
     with pytest.raises(ValueError):
         sweep.run_sweep(None, [sweep.SweepPoint(L=16, G=-1.6, R=4.0)], rest_init='hot')
+
+
+def test_sim_candidates_cover_every_variant():
+    """The single-domain tuner times every launch variant, the x-shifted
+    storage (10) included, on every tile and march length."""
+    from idmx import tune
+    for D, tiles in ((2, ((64, 1), (128, 1), (256, 1))),
+                     (3, ((32, 8), (64, 4), (128, 2), (256, 2)))):
+        cands = tune.sim_candidates(D, 256)
+        assert len(cands) == len(set(cands)) == len(tiles) * 3 * 11
+        assert {c[3] for c in cands} == set(range(11))
+        assert {c[:2] for c in cands if c[3] == 10} == set(tiles)
+    assert {c[2] for c in tune.sim_candidates(3, 40)} == {32}
+    assert {c[2] for c in tune.sim_candidates(2, 17)} == {17}
