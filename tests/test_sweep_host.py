@@ -1,6 +1,7 @@
-"""Host logic of the sweep driver and the autotuners that needs no GPU: sweep point sampling,
-the droplet end-state classifier, the Laplace pressure probe and the batch
-autotuner's candidate launches."""
+"""Host logic of the sweep driver and the autotuners that needs no GPU:
+sweep point sampling, the droplet end-state classifier, the Laplace
+pressure probe, and the candidate launches of the batch and single-domain
+autotuners."""
 import numpy as np
 import pytest
 
