@@ -312,10 +312,12 @@ struct Impl {
     // step in it, convert back (nsteps + 2 buffer swaps: same parity as
     // nsteps, so the caller's pointer bookkeeping is unchanged)
     const bool pull = COLL < 3 && !twop && !fwalls && pull_form(s->hist) && nsteps > 0;
-    // variant 10 keeps the populations in the x-shifted storage between its
-    // steps (k_fused_dma<..., XSH>): the first of the nxsh steps reads the
-    // streamed storage, the last writes it (the step that writes n/psi/u
-    // runs on the streamed storage), so a run of one step is variant 9
+    // variant 10, the fused step with walls and the fixed Shan-Chen-forcing /
+    // isotropy-filter launch keep the populations in the x-shifted storage
+    // between their steps (k_fused_dma<..., XSH>): the first of the nxsh
+    // steps reads the streamed storage, the last writes it (the step that
+    // writes n/psi/u runs on the streamed storage), so a run of one step
+    // never leaves it
     const bool xs = fwalls || (COLL < 3 ? s->hist == 10 : (kFixedDma2 || kFixedDma3));
     const int nxsh = xs && !twop ? nsteps - (wf ? 1 : 0) : 0;
     if constexpr (COLL < 3) if (pull) {
